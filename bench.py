@@ -1,0 +1,174 @@
+"""Benchmark of the zldsp::fft hot path on one MI355X.
+
+Workload (one "step"): one forward pass of 16384 complex FP32 CFFTs of
+N = 4096 points, AoS in and AoS out (1 GiB of algorithmic HBM traffic:
+16 B per point, one read of the input and one write of the output), with the
+inputs resident in HBM.  The step is one launch of the LDS-resident kernel
+through the C-ABI (zlfft_cfft_exec) on torch's current stream.
+
+Printed: ONE JSON line with whole-job throughput in GB/s of algorithmic
+traffic, the kernel's roofline fraction against the 8 TB/s HBM peak (average
+launch duration from HIP events bracketing every launch in the timed region),
+the accuracy of the timed output against the FP64 oracle, and a CPU baseline:
+the oracle's FP32 restatement of the reference algorithm, timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+ORDER, BATCH = 12, 16384
+N = 1 << ORDER
+BYTES_PER_STEP = 16 * N * BATCH  # read N complex64 + write N complex64, per transform
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cfft4096_b16384.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(seconds: float) -> dict:
+    """The oracle's FP32 restatement of the reference CFFT<float> on this host:
+    a contiguous slice of the batch per thread, one workspace per thread."""
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sample = 2048  # transforms per timed run (1/8 of the batch)
+    x = np.random.default_rng(0).standard_normal((sample, 2 * N)).astype(np.float32)
+    y = np.empty_like(x)
+    oracle.cfft_batch_raw(ORDER, x, y, True, "f32", threads)  # warm-up
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(times) < 3:
+        t0 = time.perf_counter()
+        oracle.cfft_batch_raw(ORDER, x, y, True, "f32", threads)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    l1, sw = oracle.switch_order("f32")
+    path = "stockham" if ORDER < sw else "dif-macro"
+    return {
+        "value": round(16 * N * sample / t / 1e9, 3),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample} x 4096-point CFFT<float> forward, AoS, median of {len(times)} runs; "
+                  f"oracle FP32 restatement of the reference ({path} plan, switch order {sw})",
+        "transforms_per_s": round(sample / t, 1),
+    }
+
+
+def traffic_bytes():
+    """HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel of known size), if profiled."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the batch of independent transforms "
+              "would shard with no communication, see the sharding paragraph of the design notes.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import oracle
+    import zldsp_fft_amd as zf
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    plan = zf.CFFT(ORDER)
+    g = torch.Generator(device=dev).manual_seed(4)
+    x = torch.randn((BATCH, N), dtype=torch.complex64, device=dev, generator=g)
+    y = torch.empty_like(x)
+    stream = torch.cuda.current_stream(dev)
+    lib = zf.load_library()
+    xp, yp, h, sp = x.data_ptr(), y.data_ptr(), plan._h, stream.cuda_stream
+
+    def step():
+        rc = lib.zlfft_cfft_exec(h, zf.FORWARD, zf.AOS, xp, None, zf.AOS, yp, None, BATCH, sp)
+        if rc:
+            raise RuntimeError(lib.zlfft_last_error().decode())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: K steps, each launch bracketed by events on the launch stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        step()
+        ev[i][1].record(stream)
+    t_stop.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_stop)
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    ms_per_step = total_ms / args.steps
+    kern_ms = float(np.mean(launch_ms))
+
+    # accuracy of the timed output against the FP64 oracle (sampled transforms)
+    idx = list(range(0, BATCH, BATCH // 16))
+    xs = x[idx].cpu().numpy().astype(np.complex128)
+    ys = y[idx].cpu().numpy().astype(np.complex128)
+    ref = np.stack([oracle.cfft(r, True, "f64") for r in xs])
+    rel = float((np.linalg.norm(ys - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
+
+    gbs = BYTES_PER_STEP / (ms_per_step * 1e-3) / 1e9
+    achieved = BYTES_PER_STEP / (kern_ms * 1e-3) / 1e9
+    out = {
+        "metric": "batched FFT GB/s (% HBM peak) at rel-L2 <=1e-6; MFMA-split vs plain FP32 GPU",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cfft_f32_aos_n4096_batch16384_forward", "order": ORDER, "batch": BATCH,
+                   "layout": "aos->aos", "path": "plain-f32-lds-stockham"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes()},
+        "rel_l2_max_vs_fp64": rel,
+        "transforms_per_s": round(BATCH / (ms_per_step * 1e-3), 1),
+        "gflops_5nlog2n": round(5 * N * ORDER * BATCH / (ms_per_step * 1e-3) / 1e9, 1),
+        "kernel_ms_avg": round(kern_ms, 5),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    if rel > 1e-6:
+        out["value_invalid_reason"] = "accuracy above rel-L2 1e-6"
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/*
+ * libzlfft: C-ABI of the MI355X (gfx950) FFT engine behind the zldsp::fft
+ * CFFT<F> / RFFT<F> API.
+ *
+ * The reference (ZL-Audio/zldsp_fft) is a header-only C++ library with no C
+ * ABI; its interface is the two class templates in src/zldsp_fft_cfft.hpp and
+ * src/zldsp_fft_rfft.hpp.  Each entry point below names the reference member
+ * it replaces.  Plain pointers and sizes only; `stream` is a hipStream_t
+ * passed as void* (NULL = the default stream).
+ *
+ * Semantics (identical to the reference):
+ *   forward  X[k] = sum_n x[n] e^{-2 pi i n k / N}, unnormalised;
+ *   backward = unnormalised inverse, so backward(forward(x)) = N x (CFFT);
+ *   RFFT forward: N reals -> N/2+1 bins, bins 0 and N/2 have imag exactly 0;
+ *   RFFT backward: N/2+1 bins -> N reals = (N/2) irfft(X), with the
+ *     imaginary parts of bins 0 and N/2 ignored;
+ *   outputs in natural order; inputs are never written.
+ * Layouts: ZLFFT_AOS = interleaved (std::complex<F>*), ZLFFT_SOA = separate
+ * real and imaginary arrays (std::array<F*, 2>), reference
+ * src/common/zldsp_fft_common_structure.hpp:8-33.
+ *
+ * Batched device entry points (`*_exec`) take device pointers, run
+ * asynchronously on `stream`, and process `batch` transforms stored back to
+ * back (CFFT: N complex per transform; RFFT real side: N floats; RFFT complex
+ * side: N/2+1 complex; squared magnitude: N/2+1 floats).
+ * Synchronous entry points (no `_exec` suffix) process one transform, accept
+ * host or device pointers, and return when the result is in `out`; they are
+ * what the drop-in C++ classes in zldsp_fft_cfft.hpp / zldsp_fft_rfft.hpp call.
+ *
+ * Errors: every int-returning function returns ZLFFT_OK (0) or a ZLFFT_ERR_*
+ * code; zlfft_last_error() describes the last failure of the calling thread.
+ * The reference has no error paths (its invalid inputs are undefined
+ * behaviour); here they are reported.
+ *
+ * Threading: a plan may be used by one thread at a time (the reference's
+ * instances are not shareable either, src/common/zldsp_fft_common_init.hpp:24-43);
+ * distinct plans are independent.
+ */
+#ifndef ZLFFT_H
+#define ZLFFT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZLFFT_OK 0
+#define ZLFFT_ERR_INVALID_ORDER 1
+#define ZLFFT_ERR_NULL 2
+#define ZLFFT_ERR_ALLOC 3
+#define ZLFFT_ERR_HIP 4
+#define ZLFFT_ERR_INVALID_ARG 5
+#define ZLFFT_ERR_UNSUPPORTED 6
+
+/* plan kinds */
+#define ZLFFT_KIND_CFFT 0
+#define ZLFFT_KIND_RFFT 1
+
+/* arithmetic of a plan */
+#define ZLFFT_PREC_F32 0
+
+/* user layouts */
+#define ZLFFT_AOS 0
+#define ZLFFT_SOA 1
+
+/* CFFT directions */
+#define ZLFFT_FORWARD 0
+#define ZLFFT_BACKWARD 1
+
+/* largest supported order: CFFT 2^ZLFFT_MAX_CFFT_ORDER points */
+#define ZLFFT_MAX_CFFT_ORDER 14
+
+typedef struct zlfft_plan_s *zlfft_plan;
+
+/* Replaces the constructors CFFT<F>::CFFT(size_t order)
+ * (reference src/zldsp_fft_cfft.hpp:19-23) and RFFT<F>::RFFT(size_t order)
+ * (reference src/zldsp_fft_rfft.hpp:24-53).  Builds the device plan: kernel
+ * choice and FP32 twiddle tables generated on the GPU in FP64.
+ * kind: ZLFFT_KIND_CFFT (size 2^order complex) or ZLFFT_KIND_RFFT
+ * (size 2^order real, order >= 1). */
+int zlfft_plan_create(zlfft_plan *plan, int kind, int precision, size_t order);
+
+/* Replaces the destructor (device memory is released). */
+int zlfft_plan_destroy(zlfft_plan plan);
+
+/* Replace get_size() / get_order() (reference src/zldsp_fft_cfft.hpp:25-31,
+ * src/zldsp_fft_rfft.hpp:55-61).  0 for a NULL plan. */
+size_t zlfft_plan_get_size(zlfft_plan plan);
+size_t zlfft_plan_get_order(zlfft_plan plan);
+
+/* Batched device-pointer CFFT: the eight CFFT<F>::forward/backward overloads
+ * (reference src/zldsp_fft_cfft.hpp:38-103) over `batch` transforms.
+ * in1 / out1 are the imaginary arrays for ZLFFT_SOA and ignored for AoS. */
+int zlfft_cfft_exec(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
+                    int out_layout, void *out0, void *out1, size_t batch, void *stream);
+
+/* Batched device-pointer RFFT<F>::forward (reference src/zldsp_fft_rfft.hpp:68-79). */
+int zlfft_rfft_forward_exec(zlfft_plan plan, const float *in, int out_layout, void *out0, void *out1,
+                            size_t batch, void *stream);
+
+/* Batched device-pointer RFFT<F>::forward_sqr_mag (reference src/zldsp_fft_rfft.hpp:104-109). */
+int zlfft_rfft_forward_sqr_mag_exec(zlfft_plan plan, const float *in, float *out, size_t batch, void *stream);
+
+/* Batched device-pointer RFFT<F>::backward (reference src/zldsp_fft_rfft.hpp:86-97). */
+int zlfft_rfft_backward_exec(zlfft_plan plan, int in_layout, const void *in0, const void *in1, float *out,
+                             size_t batch, void *stream);
+
+/* Synchronous single-transform forms of the four calls above for host or
+ * device pointers (the reference's calling convention: the call returns with
+ * the result in out). */
+int zlfft_cfft_execute(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
+                       int out_layout, void *out0, void *out1);
+int zlfft_rfft_forward(zlfft_plan plan, const float *in, int out_layout, void *out0, void *out1);
+int zlfft_rfft_forward_sqr_mag(zlfft_plan plan, const float *in, float *out);
+int zlfft_rfft_backward(zlfft_plan plan, int in_layout, const void *in0, const void *in1, float *out);
+
+/* Copies the plan's RFFT post/pre twiddles (wc, ws) for k = 1..N/4 as 2*(N/4)
+ * floats to host memory (reference src/common/zldsp_fft_common_init.hpp:347-388);
+ * for tests that compare them with the FP64 oracle. */
+int zlfft_plan_copy_rfft_twiddles(zlfft_plan plan, float *host_out, size_t n_floats);
+
+/* Description of the last error on the calling thread ("" if none). */
+const char *zlfft_last_error(void);
+
+/* Library version as MAJOR * 10000 + MINOR * 100 + PATCH. */
+int zlfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZLFFT_H */

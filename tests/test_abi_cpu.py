@@ -1,0 +1,77 @@
+"""CPU-only checks of the C-ABI boundary: the library loads, exports every
+function include/zlfft.h declares, and rejects invalid arguments before any
+GPU work (the reference has no error paths; its invalid inputs are UB)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import zldsp_fft_amd as zf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    hdr = open(os.path.join(ROOT, "include", "zlfft.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    return sorted(set(re.findall(r"\b(zlfft_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = zf.load_library()
+    names = _declared()
+    assert "zlfft_plan_create" in names and len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(zf.EXPORTS)
+
+
+def test_version():
+    assert zf.load_library().zlfft_version() >= 100
+
+
+def test_constants_match_header():
+    hdr = open(os.path.join(ROOT, "include", "zlfft.h")).read()
+    consts = dict(re.findall(r"#define (ZLFFT_[A-Z0-9_]+) (\d+)", hdr))
+    assert int(consts["ZLFFT_AOS"]) == zf.AOS and int(consts["ZLFFT_SOA"]) == zf.SOA
+    assert int(consts["ZLFFT_FORWARD"]) == zf.FORWARD and int(consts["ZLFFT_BACKWARD"]) == zf.BACKWARD
+    assert int(consts["ZLFFT_KIND_CFFT"]) == zf.KIND_CFFT and int(consts["ZLFFT_KIND_RFFT"]) == zf.KIND_RFFT
+    assert int(consts["ZLFFT_MAX_CFFT_ORDER"]) == zf.MAX_CFFT_ORDER
+
+
+def _create(kind, prec, order):
+    lib = zf.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.zlfft_plan_create(ctypes.byref(h), kind, prec, order)
+    return rc, h, lib.zlfft_last_error().decode()
+
+
+@pytest.mark.parametrize("kind,prec,order,code", [
+    (zf.KIND_RFFT, 0, 0, 1),           # RFFT order 0: the reference underflows rfft_order - 1
+    (zf.KIND_CFFT, 0, 40, 1),          # beyond the supported maximum
+    (zf.KIND_RFFT, 0, 40, 1),
+    (7, 0, 10, 5),                     # unknown kind
+    (zf.KIND_CFFT, 9, 10, 6),          # unknown precision
+])
+def test_plan_create_rejects_invalid_arguments(kind, prec, order, code):
+    rc, h, msg = _create(kind, prec, order)
+    assert rc == code and not h.value and msg
+
+
+def test_null_arguments_are_reported():
+    lib = zf.load_library()
+    assert lib.zlfft_plan_create(None, 0, 0, 4) == 2
+    assert lib.zlfft_plan_destroy(None) == 2
+    assert lib.zlfft_plan_get_size(None) == 0 and lib.zlfft_plan_get_order(None) == 0
+    assert lib.zlfft_cfft_exec(None, 0, 0, None, None, 0, None, None, 1, None) == 2
+    assert lib.zlfft_rfft_forward(None, None, 0, None, None) == 2
+    assert "NULL" in lib.zlfft_last_error().decode()
+
+
+def test_python_mirror_validates_order():
+    with pytest.raises(ValueError):
+        zf.CFFT(-1)
+    with pytest.raises(zf.ZlfftError) as e:
+        zf.RFFT(0)
+    assert e.value.code == 1

@@ -1,0 +1,262 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle.
+
+Tolerance: relative L2 per transform <= 1e-6 against the FP64 oracle (the
+reference's CFFT<double> restated) and against the FP32 oracle (the
+reference's CFFT<float> / RFFT<float> restated).  Exact checks where the
+contract is exact: imag of RFFT bins 0 and N/2 is 0.0, AoS/SoA/host/device
+variants of one transform agree bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import zldsp_fft_amd as zf
+from zldsp_fft_amd import signals
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+LAYOUTS = [(zf.AOS, zf.AOS), (zf.AOS, zf.SOA), (zf.SOA, zf.AOS), (zf.SOA, zf.SOA)]
+
+
+def rel(y, ref):
+    return signals.rel_l2(y, ref)
+
+
+def to_dev_complex(x):
+    """complex (B, N) -> device buffer in AoS (complex64 tensor)."""
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.complex64)).cuda()
+
+
+def make_in(x, layout):
+    if layout == zf.AOS:
+        return to_dev_complex(x)
+    return (torch.from_numpy(np.ascontiguousarray(x.real, dtype=np.float32)).cuda(),
+            torch.from_numpy(np.ascontiguousarray(x.imag, dtype=np.float32)).cuda())
+
+
+def make_out(shape, layout):
+    if layout == zf.AOS:
+        return torch.full(shape, complex(np.nan, np.nan), dtype=torch.complex64, device="cuda")
+    return (torch.full(shape, np.nan, dtype=torch.float32, device="cuda"),
+            torch.full(shape, np.nan, dtype=torch.float32, device="cuda"))
+
+
+def fetch(buf):
+    torch.cuda.synchronize()
+    if isinstance(buf, tuple):
+        return buf[0].cpu().numpy().astype(np.float64) + 1j * buf[1].cpu().numpy().astype(np.float64)
+    return buf.cpu().numpy().astype(np.complex128)
+
+
+def oracle_rows(x, forward, precision):
+    return np.stack([oracle.cfft(r, forward, precision) for r in x])
+
+
+@pytest.mark.parametrize("order", range(0, zf.MAX_CFFT_ORDER + 1))
+def test_cfft_orders_layouts_directions(order):
+    n = 1 << order
+    batch = 3  # odd: exercises a partly filled workgroup
+    x = signals.complex_uniform(1000 + order, batch, n).astype(np.complex64).astype(np.complex128)
+    plan = zf.CFFT(order)
+    assert plan.get_size() == n and plan.get_order() == order
+    for forward in (True, False):
+        ref64 = oracle_rows(x, forward, "f64")
+        ref32 = oracle_rows(x, forward, "f32")
+        outs = []
+        for lin, lout in LAYOUTS:
+            xi, yo = make_in(x, lin), make_out((batch, n), lout)
+            (plan.forward if forward else plan.backward)(xi, yo)
+            y = fetch(yo)
+            print(f"order={order} fwd={forward} lin={lin} lout={lout} "
+                  f"relL2_f64={rel(y, ref64).max():.3e} relL2_f32ref={rel(y, ref32).max():.3e}", flush=True)
+            assert np.all(np.isfinite(y))
+            assert rel(y, ref64).max() <= TOL
+            assert rel(y, ref32).max() <= TOL
+            outs.append(y)
+        for y in outs[1:]:  # layouts move values, never change them
+            assert np.array_equal(y, outs[0])
+
+
+def test_golden_fixtures(fixtures):
+    for order in range(0, 11):
+        x = fixtures[f"cfft_x_{order}"][None, :]
+        plan = zf.CFFT(order)
+        for forward, key in ((True, "fwd"), (False, "bwd")):
+            xi, yo = make_in(x, zf.AOS), make_out(x.shape, zf.AOS)
+            (plan.forward if forward else plan.backward)(xi, yo)
+            assert rel(fetch(yo), fixtures[f"cfft_{key}_{order}"][None, :]).max() <= TOL
+    for order in range(1, 11):
+        n = 1 << order
+        plan = zf.RFFT(order)
+        x = torch.from_numpy(fixtures[f"rfft_x_{order}"].astype(np.float32)).cuda()
+        y = make_out((n // 2 + 1,), zf.AOS)
+        plan.forward(x, y)
+        assert rel(fetch(y), fixtures[f"rfft_fwd_{order}"]) <= TOL
+        b = to_dev_complex(fixtures[f"rfft_bins_{order}"])
+        r = torch.full((n,), np.nan, dtype=torch.float32, device="cuda")
+        plan.backward(b, r)
+        torch.cuda.synchronize()
+        assert rel(r.cpu().numpy().astype(np.float64), fixtures[f"rfft_bwd_{order}"]) <= TOL
+
+
+@pytest.mark.parametrize("order", [0, 1, 5, 10, 12, 14])
+def test_round_trip_is_n_times_x(order):
+    n = 1 << order
+    x = signals.complex_normal(7 + order, 5, n)
+    plan = zf.CFFT(order)
+    xi, y, z = to_dev_complex(x), make_out((5, n), zf.AOS), make_out((5, n), zf.AOS)
+    plan.forward(xi, y)
+    plan.backward(y, z)
+    xr = x.astype(np.complex64).astype(np.complex128)
+    assert rel(fetch(z), n * xr).max() <= TOL
+
+
+@pytest.mark.parametrize("order", range(1, zf.MAX_CFFT_ORDER + 2))
+def test_rfft_orders(order):
+    n, nb, batch = 1 << order, (1 << order) // 2 + 1, 3
+    x = signals.uniform(2000 + order, batch * n).reshape(batch, n).astype(np.float32).astype(np.float64)
+    bins = signals.complex_uniform(3000 + order, batch, nb).astype(np.complex64).astype(np.complex128)
+    plan = zf.RFFT(order)
+    assert plan.get_size() == n and plan.get_order() == order
+    ref64 = np.stack([oracle.rfft_forward(r, "f64") for r in x])
+    ref32 = np.stack([oracle.rfft_forward(r, "f32") for r in x])
+    xd = torch.from_numpy(x.astype(np.float32)).cuda()
+    outs = []
+    for lay in (zf.AOS, zf.SOA):
+        y = make_out((batch, nb), lay)
+        plan.forward(xd, y)
+        Y = fetch(y)
+        print(f"rfft order={order} lay={lay} relL2_f64={rel(Y, ref64).max():.3e} "
+              f"relL2_f32ref={rel(Y, ref32).max():.3e}", flush=True)
+        assert rel(Y, ref64).max() <= TOL and rel(Y, ref32).max() <= TOL
+        assert np.all(Y[:, 0].imag == 0.0) and np.all(Y[:, -1].imag == 0.0)
+        outs.append(Y)
+    assert np.array_equal(outs[0], outs[1])
+    sq = torch.full((batch, nb), np.nan, dtype=torch.float32, device="cuda")
+    plan.forward_sqr_mag(xd, sq)
+    torch.cuda.synchronize()
+    S = sq.cpu().numpy().astype(np.float64)
+    S64 = np.stack([oracle.rfft_sqr_mag(r, "f64") for r in x])
+    assert (np.linalg.norm(S - S64, axis=1) / np.linalg.norm(S64, axis=1)).max() <= 2 * TOL
+    # backward: random imag at DC / Nyquist must be ignored
+    b64 = np.stack([oracle.rfft_backward(b, n, "f64") for b in bins])
+    b32 = np.stack([oracle.rfft_backward(b, n, "f32") for b in bins])
+    res = []
+    for lay in (zf.AOS, zf.SOA):
+        r = torch.full((batch, n), np.nan, dtype=torch.float32, device="cuda")
+        plan.backward(make_in(bins, lay), r)
+        torch.cuda.synchronize()
+        R = r.cpu().numpy().astype(np.float64)
+        print(f"irfft order={order} lay={lay} relL2_f64={rel(R, b64).max():.3e}", flush=True)
+        assert rel(R, b64).max() <= TOL and rel(R, b32).max() <= TOL
+        res.append(R)
+    assert np.array_equal(res[0], res[1])
+
+
+def test_host_pointer_path_matches_device_batch():
+    """Synchronous host-buffer calls (the reference's calling convention)
+    agree bit for bit with the batched device path."""
+    order, n = 10, 1024
+    x = signals.complex_uniform(5, 2, n).astype(np.complex64)
+    plan = zf.CFFT(order)
+    yd = make_out((2, n), zf.AOS)
+    plan.forward(to_dev_complex(x), yd)
+    Yd = fetch(yd)
+    for i in range(2):
+        yh = np.zeros(n, np.complex64)
+        plan.forward(np.ascontiguousarray(x[i]), yh)
+        assert np.array_equal(yh.astype(np.complex128), Yd[i])
+        re, im = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        plan.backward(np.ascontiguousarray(x[i]), (re, im))
+        zd = make_out((1, n), zf.AOS)
+        plan.backward(to_dev_complex(x[i:i + 1]), zd)
+        assert np.array_equal((re + 1j * im).astype(np.complex128), fetch(zd)[0])
+    r = zf.RFFT(11)
+    xr = signals.uniform(9, 2048).astype(np.float32)
+    Xh = np.zeros(1025, np.complex64)
+    r.forward(xr, Xh)
+    Xd = make_out((1025,), zf.AOS)
+    r.forward(torch.from_numpy(xr).cuda(), Xd)
+    assert np.array_equal(Xh.astype(np.complex128), fetch(Xd))
+    back = np.zeros(2048, np.float32)
+    r.backward(Xh, back)
+    assert rel(back.astype(np.float64), 1024 * xr.astype(np.float64)) <= TOL
+
+
+def test_cfft_in_place():
+    order, n, batch = 12, 4096, 9
+    x = signals.complex_uniform(17, batch, n)
+    plan = zf.CFFT(order)
+    buf = to_dev_complex(x)
+    ref = make_out((batch, n), zf.AOS)
+    plan.forward(buf, ref)
+    plan.forward(buf, buf)
+    assert np.array_equal(fetch(buf), fetch(ref))
+
+
+def test_rfft_twiddles_match_fp64_oracle_rounding():
+    """The plan's FP32 (wc, ws) table, generated on the GPU in FP64, equals the
+    reference's host formula (init.hpp:374-386) rounded to float."""
+    for order in (3, 8, 11, 15):
+        n = 1 << order
+        tw = zf.RFFT(order).rfft_twiddles().astype(np.float64)
+        k = np.arange(1, n // 4 + 1)
+        a = k * (2.0 / n)
+        wc = np.array([np.float32(0.5 * oracle.cospi(v)) for v in a], dtype=np.float64)
+        ws = np.array([np.float32(0.5 * oracle.sinpi(v) + 0.5) for v in a], dtype=np.float64)
+        ulp = np.spacing(np.float32(1.0)).astype(np.float64)
+        assert np.abs(tw[:, 0] - wc).max() <= ulp and np.abs(tw[:, 1] - ws).max() <= ulp
+        print(f"rfft tw order={order}: {(tw[:, 0] != wc).sum() + (tw[:, 1] != ws).sum()} of {2 * k.size} differ by 1 ulp")
+
+
+def test_analyzer_frames_config():
+    """4096 frames x N=2048 real: Hann-windowed Gaussian noise (sigma 0.1) plus
+    three sinusoids; AoS bins and forward_sqr_mag against the reference's
+    RFFT<float> (FP32 oracle) and FP64, rel-L2 <= 1e-6 for every frame."""
+    frames, n = 4096, 2048
+    x = signals.analyzer_frames(2, frames, n).astype(np.float32)
+    plan = zf.RFFT(11)
+    xd = torch.from_numpy(x).cuda()
+    Y = make_out((frames, n // 2 + 1), zf.AOS)
+    S = torch.empty((frames, n // 2 + 1), dtype=torch.float32, device="cuda")
+    plan.forward(xd, Y)
+    plan.forward_sqr_mag(xd, S)
+    Yh, Sh = fetch(Y), S.cpu().numpy().astype(np.float64)
+    sample = np.arange(0, frames, 16)
+    x64 = x.astype(np.float64)
+    r64 = np.stack([oracle.rfft_forward(x64[i], "f64") for i in sample])
+    r32 = np.stack([oracle.rfft_forward(x64[i], "f32") for i in sample])
+    print(f"analyzer: relL2_f64 max={rel(Yh[sample], r64).max():.3e} vs RFFT<float> "
+          f"max={rel(Yh[sample], r32).max():.3e}", flush=True)
+    assert rel(Yh[sample], r64).max() <= TOL and rel(Yh[sample], r32).max() <= TOL
+    s64 = np.abs(r64) ** 2
+    assert (np.linalg.norm(Sh[sample] - s64, axis=1) / np.linalg.norm(s64, axis=1)).max() <= 2 * TOL
+    # every frame: agrees with numpy's FP64 rfft of the same FP32 input
+    full = np.fft.rfft(x64, axis=1)
+    assert rel(Yh, full).max() <= TOL
+
+
+def test_batched_4096_point_config_properties():
+    """16384 x 4096-point AoS CFFTs: every transform's round trip equals N x,
+    Parseval holds, and sampled transforms match the FP64 oracle."""
+    batch, n = 16384, 4096
+    g = torch.Generator(device="cuda").manual_seed(4)
+    x = torch.randn((batch, n), dtype=torch.complex64, device="cuda", generator=g)
+    plan = zf.CFFT(12)
+    y = torch.empty_like(x)
+    z = torch.empty_like(x)
+    plan.forward(x, y)
+    plan.backward(y, z)
+    torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(z - n * x, dim=1) / torch.linalg.vector_norm(n * x, dim=1)).max().item()
+    ex = torch.linalg.vector_norm(x, dim=1) ** 2
+    ey = torch.linalg.vector_norm(y, dim=1) ** 2
+    pars = ((ey - n * ex).abs() / ey).max().item()
+    print(f"round trip max relL2={rt:.3e}, Parseval max rel={pars:.3e}", flush=True)
+    assert rt <= TOL and pars <= 1e-5
+    idx = np.arange(0, batch, 1024)
+    xs = x[idx].cpu().numpy().astype(np.complex128)
+    ys = y[idx].cpu().numpy().astype(np.complex128)
+    assert rel(ys, oracle_rows(xs, True, "f64")).max() <= TOL

@@ -1,0 +1,321 @@
+"""zldsp_fft_amd: the zldsp::fft CFFT / RFFT API on MI355X (gfx950).
+
+Python mirror of the reference's two classes (reference
+``src/zldsp_fft_cfft.hpp:11-118`` and ``src/zldsp_fft_rfft.hpp:11-139``) over the
+C-ABI in ``include/zlfft.h`` (``libzlfft.so``, built in-tree by
+``zldsp_fft_amd/build.py``).  The same names, argument meaning and overloads:
+
+* ``CFFT(order)``: ``get_size()``, ``get_order()``, ``forward(in, out)``,
+  ``backward(in, out)``, where each buffer is AoS (a complex64 array / tensor,
+  the reference's ``std::complex<float>*``) or SoA (a 2-sequence
+  ``(real, imag)`` of float32 arrays, the reference's ``std::array<F*, 2>``).
+* ``RFFT(order)``: ``forward(real, out)``, ``backward(in, real)``,
+  ``forward_sqr_mag(real, out)``, ``get_size()``, ``get_order()``.
+
+Host buffers (numpy arrays or CPU tensors) hold one transform and the call is
+synchronous, like the reference.  CUDA (HIP) tensors may hold a batch of
+transforms back to back (``numel`` a multiple of the per-transform size); the
+whole batch runs as one launch on torch's current stream.
+
+There is no CPU fallback: if ``libzlfft.so`` is missing or no GPU is present
+the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Sequence
+
+import numpy as np
+
+__all__ = ["CFFT", "RFFT", "ZlfftError", "lib_path", "load_library", "AOS", "SOA"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libzlfft.so")
+
+# constants of include/zlfft.h
+OK = 0
+KIND_CFFT, KIND_RFFT = 0, 1
+PREC_F32 = 0
+AOS, SOA = 0, 1
+FORWARD, BACKWARD = 0, 1
+MAX_CFFT_ORDER = 14
+
+_ERRNAMES = {1: "INVALID_ORDER", 2: "NULL", 3: "ALLOC", 4: "HIP", 5: "INVALID_ARG", 6: "UNSUPPORTED"}
+
+EXPORTS = (
+    "zlfft_plan_create", "zlfft_plan_destroy", "zlfft_plan_get_size", "zlfft_plan_get_order",
+    "zlfft_cfft_exec", "zlfft_rfft_forward_exec", "zlfft_rfft_forward_sqr_mag_exec", "zlfft_rfft_backward_exec",
+    "zlfft_cfft_execute", "zlfft_rfft_forward", "zlfft_rfft_forward_sqr_mag", "zlfft_rfft_backward",
+    "zlfft_plan_copy_rfft_twiddles", "zlfft_last_error", "zlfft_version",
+)
+
+
+class ZlfftError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"zlfft error {code} ({_ERRNAMES.get(code, '?')}): {msg}")
+        self.code = code
+
+
+_LIB: ctypes.CDLL | None = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libzlfft.so and declare its signatures (no GPU work happens here)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(lib_path):
+        raise ZlfftError(6, f"{lib_path} is not built; run python -m zldsp_fft_amd.build")
+    L = ctypes.CDLL(lib_path)
+    vp, sz, ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    L.zlfft_plan_create.argtypes = [ctypes.POINTER(vp), ci, ci, sz]
+    L.zlfft_plan_destroy.argtypes = [vp]
+    L.zlfft_plan_get_size.argtypes = [vp]
+    L.zlfft_plan_get_size.restype = sz
+    L.zlfft_plan_get_order.argtypes = [vp]
+    L.zlfft_plan_get_order.restype = sz
+    L.zlfft_cfft_exec.argtypes = [vp, ci, ci, vp, vp, ci, vp, vp, sz, vp]
+    L.zlfft_rfft_forward_exec.argtypes = [vp, vp, ci, vp, vp, sz, vp]
+    L.zlfft_rfft_forward_sqr_mag_exec.argtypes = [vp, vp, vp, sz, vp]
+    L.zlfft_rfft_backward_exec.argtypes = [vp, ci, vp, vp, vp, sz, vp]
+    L.zlfft_cfft_execute.argtypes = [vp, ci, ci, vp, vp, ci, vp, vp]
+    L.zlfft_rfft_forward.argtypes = [vp, vp, ci, vp, vp]
+    L.zlfft_rfft_forward_sqr_mag.argtypes = [vp, vp, vp]
+    L.zlfft_rfft_backward.argtypes = [vp, ci, vp, vp, vp]
+    L.zlfft_plan_copy_rfft_twiddles.argtypes = [vp, vp, sz]
+    L.zlfft_last_error.restype = ctypes.c_char_p
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if name not in ("zlfft_plan_get_size", "zlfft_plan_get_order", "zlfft_last_error"):
+            fn.restype = ci
+    _LIB = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        msg = load_library().zlfft_last_error().decode(errors="replace")
+        raise ZlfftError(rc, msg)
+
+
+# ---------------------------------------------------------------- buffers
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _is_dev(x) -> bool:
+    return _is_torch(x) and x.is_cuda
+
+
+def _contig(x, name: str):
+    if _is_torch(x):
+        if not x.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+    elif not (isinstance(x, np.ndarray) and x.flags.c_contiguous):
+        raise ValueError(f"{name} must be a C-contiguous numpy array or tensor")
+    return x
+
+
+def _ptr(x) -> int:
+    return x.data_ptr() if _is_torch(x) else x.ctypes.data
+
+
+def _nbytes(x) -> int:
+    return x.numel() * x.element_size() if _is_torch(x) else x.nbytes
+
+
+def _is_soa(buf) -> bool:
+    return isinstance(buf, (tuple, list))
+
+
+def _complex_view(buf, name: str):
+    """AoS buffer: complex64 or float32 with interleaved re, im."""
+    _contig(buf, name)
+    if _is_torch(buf):
+        import torch
+        if buf.dtype not in (torch.complex64, torch.float32):
+            raise TypeError(f"{name}: AoS buffers are complex64 (or interleaved float32)")
+    elif buf.dtype not in (np.complex64, np.float32):
+        raise TypeError(f"{name}: AoS buffers are complex64 (or interleaved float32)")
+    return buf
+
+
+def _float_view(buf, name: str):
+    _contig(buf, name)
+    if _is_torch(buf):
+        import torch
+        if buf.dtype != torch.float32:
+            raise TypeError(f"{name}: float32 required")
+    elif buf.dtype != np.float32:
+        raise TypeError(f"{name}: float32 required")
+    return buf
+
+
+class _Side:
+    """One side (input or output) of a call: layout, pointers, element count."""
+
+    def __init__(self, buf, name: str):
+        if _is_soa(buf):
+            if len(buf) != 2:
+                raise ValueError(f"{name}: SoA buffers are (real, imag)")
+            re, im = _float_view(buf[0], name + "[0]"), _float_view(buf[1], name + "[1]")
+            if _nbytes(re) != _nbytes(im):
+                raise ValueError(f"{name}: real and imag sizes differ")
+            self.layout, self.p0, self.p1 = SOA, _ptr(re), _ptr(im)
+            self.ncomplex = _nbytes(re) // 4
+            self.dev = _is_dev(re) and _is_dev(im)
+            self.host = not (_is_dev(re) or _is_dev(im))
+            self.ref = (re, im)
+        else:
+            b = _complex_view(buf, name)
+            self.layout, self.p0, self.p1 = AOS, _ptr(b), None
+            self.ncomplex = _nbytes(b) // 8
+            self.dev = _is_dev(b)
+            self.host = not self.dev
+            self.ref = (b,)
+
+
+def _stream_of(*tensors) -> int:
+    import torch
+    dev = next(t for t in tensors if _is_dev(t)).device
+    return torch.cuda.current_stream(dev).cuda_stream
+
+
+def _batch(n_have: int, n_per: int, name: str) -> int:
+    if n_per == 0 or n_have % n_per != 0 or n_have == 0:
+        raise ValueError(f"{name}: holds {n_have} elements, not a positive multiple of {n_per}")
+    return n_have // n_per
+
+
+# ---------------------------------------------------------------- plans
+
+class _Plan:
+    def __init__(self, kind: int, order: int):
+        if isinstance(order, bool) or not isinstance(order, (int, np.integer)) or order < 0:
+            raise ValueError("order must be a non-negative integer")
+        self._lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self._lib.zlfft_plan_create(ctypes.byref(h), kind, PREC_F32, int(order)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.zlfft_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_size(self) -> int:
+        return int(self._lib.zlfft_plan_get_size(self._h))
+
+    def get_order(self) -> int:
+        return int(self._lib.zlfft_plan_get_order(self._h))
+
+    @property
+    def handle(self) -> int:
+        return self._h.value
+
+
+class CFFT(_Plan):
+    """``zldsp::fft::CFFT<float>`` (reference ``src/zldsp_fft_cfft.hpp:11-118``)."""
+
+    def __init__(self, order: int):
+        super().__init__(KIND_CFFT, order)
+
+    def _run(self, direction: int, in_buffer, out_buffer) -> None:
+        n = self.get_size()
+        si, so = _Side(in_buffer, "in_buffer"), _Side(out_buffer, "out_buffer")
+        if si.dev and so.dev:
+            b = _batch(si.ncomplex, n, "in_buffer")
+            if _batch(so.ncomplex, n, "out_buffer") != b:
+                raise ValueError("in_buffer and out_buffer hold different batch sizes")
+            _check(self._lib.zlfft_cfft_exec(self._h, direction, si.layout, si.p0, si.p1, so.layout, so.p0, so.p1,
+                                             b, _stream_of(*si.ref, *so.ref)))
+            return
+        if si.ncomplex != n or so.ncomplex != n:
+            raise ValueError(f"host buffers must hold exactly {n} complex values")
+        _check(self._lib.zlfft_cfft_execute(self._h, direction, si.layout, si.p0, si.p1, so.layout, so.p0, so.p1))
+
+    def forward(self, in_buffer, out_buffer) -> None:
+        """AoS/SoA -> AoS/SoA forward (reference ``src/zldsp_fft_cfft.hpp:38-95``)."""
+        self._run(FORWARD, in_buffer, out_buffer)
+
+    def backward(self, in_buffer, out_buffer) -> None:
+        """AoS/SoA -> AoS/SoA backward (reference ``src/zldsp_fft_cfft.hpp:47-103``)."""
+        self._run(BACKWARD, in_buffer, out_buffer)
+
+
+class RFFT(_Plan):
+    """``zldsp::fft::RFFT<float>`` (reference ``src/zldsp_fft_rfft.hpp:11-139``)."""
+
+    def __init__(self, order: int):
+        super().__init__(KIND_RFFT, order)
+
+    def forward(self, in_buffer, out_buffer) -> None:
+        """N reals -> N/2+1 bins, AoS or SoA (reference ``src/zldsp_fft_rfft.hpp:68-79``)."""
+        n = self.get_size()
+        nb = n // 2 + 1
+        x = _float_view(in_buffer, "in_buffer")
+        so = _Side(out_buffer, "out_buffer")
+        if _is_dev(x) and so.dev:
+            b = _batch(x.numel(), n, "in_buffer")
+            if _batch(so.ncomplex, nb, "out_buffer") != b:
+                raise ValueError("in_buffer and out_buffer hold different batch sizes")
+            _check(self._lib.zlfft_rfft_forward_exec(self._h, _ptr(x), so.layout, so.p0, so.p1, b,
+                                                     _stream_of(x, *so.ref)))
+            return
+        if _nbytes(x) != 4 * n or so.ncomplex != nb:
+            raise ValueError(f"host buffers must hold {n} reals and {nb} bins")
+        _check(self._lib.zlfft_rfft_forward(self._h, _ptr(x), so.layout, so.p0, so.p1))
+
+    def backward(self, in_buffer, out_buffer) -> None:
+        """N/2+1 bins (AoS or SoA) -> N reals (reference ``src/zldsp_fft_rfft.hpp:86-97``)."""
+        n = self.get_size()
+        nb = n // 2 + 1
+        si = _Side(in_buffer, "in_buffer")
+        y = _float_view(out_buffer, "out_buffer")
+        if si.dev and _is_dev(y):
+            b = _batch(si.ncomplex, nb, "in_buffer")
+            if _batch(y.numel(), n, "out_buffer") != b:
+                raise ValueError("in_buffer and out_buffer hold different batch sizes")
+            _check(self._lib.zlfft_rfft_backward_exec(self._h, si.layout, si.p0, si.p1, _ptr(y), b,
+                                                      _stream_of(*si.ref, y)))
+            return
+        if si.ncomplex != nb or _nbytes(y) != 4 * n:
+            raise ValueError(f"host buffers must hold {nb} bins and {n} reals")
+        _check(self._lib.zlfft_rfft_backward(self._h, si.layout, si.p0, si.p1, _ptr(y)))
+
+    def forward_sqr_mag(self, in_buffer, out_buffer) -> None:
+        """N reals -> N/2+1 squared magnitudes (reference ``src/zldsp_fft_rfft.hpp:104-109``)."""
+        n = self.get_size()
+        nb = n // 2 + 1
+        x = _float_view(in_buffer, "in_buffer")
+        y = _float_view(out_buffer, "out_buffer")
+        if _is_dev(x) and _is_dev(y):
+            b = _batch(x.numel(), n, "in_buffer")
+            if _batch(y.numel(), nb, "out_buffer") != b:
+                raise ValueError("in_buffer and out_buffer hold different batch sizes")
+            _check(self._lib.zlfft_rfft_forward_sqr_mag_exec(self._h, _ptr(x), _ptr(y), b, _stream_of(x, y)))
+            return
+        if _nbytes(x) != 4 * n or _nbytes(y) != 4 * nb:
+            raise ValueError(f"host buffers must hold {n} reals and {nb} floats")
+        _check(self._lib.zlfft_rfft_forward_sqr_mag(self._h, _ptr(x), _ptr(y)))
+
+    def rfft_twiddles(self) -> np.ndarray:
+        """The plan's (wc, ws) table for k = 1..N/4, copied to the host."""
+        m = self.get_size() // 4
+        out = np.zeros(2 * m, dtype=np.float32)
+        _check(self._lib.zlfft_plan_copy_rfft_twiddles(self._h, out.ctypes.data, out.size))
+        return out.reshape(m, 2)

@@ -1,0 +1,71 @@
+"""Build recipe for libzlfft.so (hipcc, gfx950).
+
+Compiles every kernel unit in parallel and links ``zldsp_fft_amd/libzlfft.so``
+in-tree, plus the drop-in C++ example/test program.  Run as
+``python -m zldsp_fft_amd.build`` or through ``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+INCLUDE = os.path.join(ROOT, "include")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libzlfft.so")
+MAX_LDS_ORDER = 14
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    # packed FP32 (v_pk_*_f32) is kept off for every kernel
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", INCLUDE,
+    "-Wall",
+]
+
+
+def _run(cmd: list[str]) -> tuple[list[str], int, str]:
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    # hipcc's host pass warns that it does not know the packed-fp32 feature
+    err = "\n".join(l for l in (p.stdout + p.stderr).splitlines() if "packed-fp32-ops" not in l)
+    return cmd, p.returncode, err
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    units = []
+    for n in range(MAX_LDS_ORDER + 1):
+        units.append((os.path.join(CSRC, "zlfft_order.hip"), os.path.join(OBJ, f"order{n}.o"), [f"-DZL_LOGN={n}"]))
+    units.append((os.path.join(CSRC, "zlfft_api.hip"), os.path.join(OBJ, "api.o"), []))
+    cmds = [[HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd, rc, err in ex.map(_run, cmds):
+            if verbose or rc:
+                print(" ".join(cmd), file=sys.stderr)
+                if err.strip():
+                    print(err, file=sys.stderr)
+            if rc:
+                raise RuntimeError(f"hipcc failed ({rc}) for {cmd[-3]}")
+    objs = [obj for _, obj, _ in units]
+    tmp = LIB + ".tmp"
+    cmd, rc, err = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    if rc:
+        print(err, file=sys.stderr)
+        raise RuntimeError("link of libzlfft.so failed")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

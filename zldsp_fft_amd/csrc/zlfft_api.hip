@@ -1,0 +1,377 @@
+// C-ABI of libzlfft: device plans (kernel choice + twiddles generated on the
+// GPU in FP64), batched asynchronous execution on device pointers, and the
+// synchronous single-transform calls used by the drop-in C++ classes.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/zlfft.h"
+#include "zlfft_registry.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(ZLFFT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define ZL_HIP(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+zl::OrderTable g_tab[zl::kMaxLdsOrder + 1];
+std::once_flag g_tab_once;
+
+void fill_tables() {
+    zl_fill_order_0(g_tab[0]);
+    zl_fill_order_1(g_tab[1]);
+    zl_fill_order_2(g_tab[2]);
+    zl_fill_order_3(g_tab[3]);
+    zl_fill_order_4(g_tab[4]);
+    zl_fill_order_5(g_tab[5]);
+    zl_fill_order_6(g_tab[6]);
+    zl_fill_order_7(g_tab[7]);
+    zl_fill_order_8(g_tab[8]);
+    zl_fill_order_9(g_tab[9]);
+    zl_fill_order_10(g_tab[10]);
+    zl_fill_order_11(g_tab[11]);
+    zl_fill_order_12(g_tab[12]);
+    zl_fill_order_13(g_tab[13]);
+    zl_fill_order_14(g_tab[14]);
+}
+
+// Stockham twiddles of one pass: out[(m - 1) w + k] = W_{r w}^{m k}, computed
+// in FP64 (exact dyadic angle, sincospi) and rounded to FP32, as the reference
+// computes its tables in double and casts (src/common/zldsp_fft_common_init.hpp:84-90).
+__global__ void gen_pass_twiddles(float2* out, int lr, int lw, int count) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const int w = 1 << lw;
+    const long long m = idx / w + 1, k = idx % w;
+    const long long rw = 1ll << (lr + lw);
+    const long long e = (m * k) & (rw - 1);
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)rw, &s, &c);
+    out[idx] = make_float2((float)c, (float)s);
+}
+
+// RFFT pre/post twiddles, reference src/common/zldsp_fft_common_init.hpp:374-386:
+// a = k * (2 / N); wc = (F)(0.5 cospi(a)), ws = (F)(0.5 sinpi(a) + 0.5), k = 1..N/4.
+__global__ void gen_rfft_twiddles(float2* out, long long nreal, int count) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const double a = (double)(idx + 1) * (2.0 / (double)nreal);
+    double s, c;
+    sincospi(a, &s, &c);
+    out[idx] = make_float2((float)(0.5 * c), (float)(0.5 * s + 0.5));
+}
+
+}  // namespace
+
+struct zlfft_plan_s {
+    int kind = ZLFFT_KIND_CFFT;
+    int precision = ZLFFT_PREC_F32;
+    int order = 0;   // user order
+    int corder = 0;  // inner CFFT order
+    int device = 0;
+    const zl::OrderTable* tab = nullptr;
+    float2* d_tw = nullptr;
+    float2* d_rtw = nullptr;
+    int n_rtw = 0;
+    // synchronous calls on host pointers: staging buffers and a stream
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+int check_plan(zlfft_plan p) {
+    if (!p) return fail(ZLFFT_ERR_NULL, "plan is NULL");
+    return ZLFFT_OK;
+}
+
+int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
+           size_t batch, int backward, void* stream) {
+    const zl::KernelEntry& e = (*p->tab)[kind][lin][lout];
+    if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no kernel for this kind/layout combination");
+    if (batch == 0) return ZLFFT_OK;
+    zl::KArgs a{};
+    a.in0 = in0;
+    a.in1 = in1;
+    a.out0 = out0;
+    a.out1 = out1;
+    a.tw = p->d_tw;
+    a.rtw = p->d_rtw;
+    a.batch = (long long)batch;
+    a.backward = backward;
+    const size_t grid = (batch + e.tpw - 1) / e.tpw;
+    if (grid > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
+    void* params[] = {&a};
+    ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)grid), dim3(e.wg), params, 0,
+                           reinterpret_cast<hipStream_t>(stream)));
+    return ZLFFT_OK;
+}
+
+bool valid_layout(int l) { return l == ZLFFT_AOS || l == ZLFFT_SOA; }
+
+// true when ptr is device (or managed) memory the GPU can address directly
+bool is_device_ptr(const void* ptr) {
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    hipError_t e = hipPointerGetAttributes(&attr, ptr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int ensure_stage(zlfft_plan p, size_t bytes) {
+    if (p->stage_bytes >= bytes) return ZLFFT_OK;
+    if (p->d_stage) {
+        (void)hipFree(p->d_stage);
+        p->d_stage = nullptr;
+        p->stage_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p->d_stage, bytes);
+    if (e != hipSuccess) return fail(ZLFFT_ERR_ALLOC, std::string("staging hipMalloc: ") + hipGetErrorString(e));
+    p->stage_bytes = bytes;
+    return ZLFFT_OK;
+}
+
+// Host/device buffer description for the synchronous calls
+struct Buf {
+    const void* p0;
+    const void* p1;
+    size_t bytes0, bytes1;  // bytes of each array
+};
+
+// Runs `body(in0, in1, out0, out1)` on device pointers, staging host buffers.
+template <class Body>
+int run_sync(zlfft_plan p, Buf in, Buf out, Body body) {
+    const bool dev = is_device_ptr(in.p0) && (!in.p1 || is_device_ptr(in.p1)) && is_device_ptr(out.p0) &&
+                     (!out.p1 || is_device_ptr(out.p1));
+    if (dev) {
+        int rc = body(in.p0, in.p1, const_cast<void*>(out.p0), const_cast<void*>(out.p1), p->stream);
+        if (rc) return rc;
+        ZL_HIP(hipStreamSynchronize(p->stream));
+        return ZLFFT_OK;
+    }
+    const size_t total = in.bytes0 + in.bytes1 + out.bytes0 + out.bytes1;
+    int rc = ensure_stage(p, total + 256);
+    if (rc) return rc;
+    char* base = static_cast<char*>(p->d_stage);
+    char* di0 = base;
+    char* di1 = di0 + in.bytes0;
+    char* do0 = di1 + in.bytes1;
+    char* do1 = do0 + out.bytes0;
+    ZL_HIP(hipMemcpyAsync(di0, in.p0, in.bytes0, hipMemcpyDefault, p->stream));
+    if (in.p1) ZL_HIP(hipMemcpyAsync(di1, in.p1, in.bytes1, hipMemcpyDefault, p->stream));
+    rc = body(di0, in.p1 ? di1 : nullptr, do0, out.p1 ? do1 : nullptr, p->stream);
+    if (rc) return rc;
+    ZL_HIP(hipMemcpyAsync(const_cast<void*>(out.p0), do0, out.bytes0, hipMemcpyDefault, p->stream));
+    if (out.p1) ZL_HIP(hipMemcpyAsync(const_cast<void*>(out.p1), do1, out.bytes1, hipMemcpyDefault, p->stream));
+    ZL_HIP(hipStreamSynchronize(p->stream));
+    return ZLFFT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* zlfft_last_error(void) { return g_err.c_str(); }
+
+int zlfft_version(void) { return 100; }
+
+int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
+    g_err.clear();
+    if (!plan) return fail(ZLFFT_ERR_NULL, "plan out-pointer is NULL");
+    *plan = nullptr;
+    if (kind != ZLFFT_KIND_CFFT && kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "unknown plan kind");
+    if (precision != ZLFFT_PREC_F32) return fail(ZLFFT_ERR_UNSUPPORTED, "only ZLFFT_PREC_F32 is supported");
+    if (kind == ZLFFT_KIND_RFFT && order < 1)
+        return fail(ZLFFT_ERR_INVALID_ORDER, "RFFT order must be >= 1 (the reference underflows at 0)");
+    const size_t corder = kind == ZLFFT_KIND_RFFT ? order - 1 : order;
+    if (corder > (size_t)zl::kMaxLdsOrder)
+        return fail(ZLFFT_ERR_INVALID_ORDER, "order " + std::to_string(order) + " exceeds the supported maximum");
+    std::call_once(g_tab_once, fill_tables);
+
+    zlfft_plan p = new (std::nothrow) zlfft_plan_s();
+    if (!p) return fail(ZLFFT_ERR_ALLOC, "plan allocation failed");
+    p->kind = kind;
+    p->precision = precision;
+    p->order = (int)order;
+    p->corder = (int)corder;
+    p->tab = &g_tab[corder];
+    auto cleanup = [&](int rc) {
+        zlfft_plan_destroy(p);
+        return rc;
+    };
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipGetDevice"));
+    e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreateWithFlags"));
+
+    const zl::KernelEntry& ent = (*p->tab)[zl::C2C][0][0];
+    if (ent.tw_total > 0) {
+        e = hipMalloc(&p->d_tw, sizeof(float2) * (size_t)ent.tw_total);
+        if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
+        size_t off = 0;
+        for (int q = 1; q < ent.np; ++q) {
+            const int cnt = ((1 << ent.lr[q]) - 1) << ent.lw[q];
+            gen_pass_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_tw + off, ent.lr[q], ent.lw[q], cnt);
+            off += (size_t)cnt;
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
+    }
+    if (kind == ZLFFT_KIND_RFFT) {
+        const long long nreal = 1ll << order;
+        p->n_rtw = (int)(nreal / 4);
+        if (p->n_rtw > 0) {
+            e = hipMalloc(&p->d_rtw, sizeof(float2) * (size_t)p->n_rtw);
+            if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "rfft twiddle hipMalloc failed"));
+            gen_rfft_twiddles<<<(p->n_rtw + 255) / 256, 256, 0, p->stream>>>(p->d_rtw, nreal, p->n_rtw);
+            e = hipGetLastError();
+            if (e != hipSuccess) return cleanup(hip_fail(e, "rfft twiddle generation"));
+        }
+    }
+    e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "plan synchronize"));
+    *plan = p;
+    return ZLFFT_OK;
+}
+
+int zlfft_plan_destroy(zlfft_plan p) {
+    if (!p) return fail(ZLFFT_ERR_NULL, "plan is NULL");
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->d_tw) (void)hipFree(p->d_tw);
+    if (p->d_rtw) (void)hipFree(p->d_rtw);
+    if (p->d_stage) (void)hipFree(p->d_stage);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return ZLFFT_OK;
+}
+
+size_t zlfft_plan_get_size(zlfft_plan p) { return p ? (size_t)1 << p->order : 0; }
+
+size_t zlfft_plan_get_order(zlfft_plan p) { return p ? (size_t)p->order : 0; }
+
+int zlfft_cfft_exec(zlfft_plan p, int direction, int in_layout, const void* in0, const void* in1, int out_layout,
+                    void* out0, void* out1, size_t batch, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_CFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not a CFFT plan");
+    if (direction != ZLFFT_FORWARD && direction != ZLFFT_BACKWARD)
+        return fail(ZLFFT_ERR_INVALID_ARG, "direction must be ZLFFT_FORWARD or ZLFFT_BACKWARD");
+    if (!valid_layout(in_layout) || !valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch && (!in0 || !out0 || (in_layout == ZLFFT_SOA && !in1) || (out_layout == ZLFFT_SOA && !out1)))
+        return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    return launch(p, zl::C2C, in_layout, out_layout, in0, in1, out0, out1, batch, direction == ZLFFT_BACKWARD,
+                  stream);
+}
+
+int zlfft_rfft_forward_exec(zlfft_plan p, const float* in, int out_layout, void* out0, void* out1, size_t batch,
+                            void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch && (!in || !out0 || (out_layout == ZLFFT_SOA && !out1))) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    return launch(p, zl::R2C, 0, out_layout, in, nullptr, out0, out1, batch, 0, stream);
+}
+
+int zlfft_rfft_forward_sqr_mag_exec(zlfft_plan p, const float* in, float* out, size_t batch, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (batch && (!in || !out)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    return launch(p, zl::R2C_SQR, 0, 0, in, nullptr, out, nullptr, batch, 0, stream);
+}
+
+int zlfft_rfft_backward_exec(zlfft_plan p, int in_layout, const void* in0, const void* in1, float* out, size_t batch,
+                             void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!valid_layout(in_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch && (!in0 || !out || (in_layout == ZLFFT_SOA && !in1))) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    return launch(p, zl::C2R, in_layout, 0, in0, in1, out, nullptr, batch, 1, stream);
+}
+
+int zlfft_cfft_execute(zlfft_plan p, int direction, int in_layout, const void* in0, const void* in1, int out_layout,
+                       void* out0, void* out1) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_CFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not a CFFT plan");
+    if (!valid_layout(in_layout) || !valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (!in0 || !out0 || (in_layout == ZLFFT_SOA && !in1) || (out_layout == ZLFFT_SOA && !out1))
+        return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    const size_t n = (size_t)1 << p->order;
+    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 8 * n : 4 * n,
+                 in_layout == ZLFFT_SOA ? 4 * n : 0};
+    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 8 * n : 4 * n,
+                 out_layout == ZLFFT_SOA ? 4 * n : 0};
+    return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void* o1, hipStream_t s) {
+        return zlfft_cfft_exec(p, direction, in_layout, i0, i1, out_layout, o0, o1, 1, s);
+    });
+}
+
+int zlfft_rfft_forward(zlfft_plan p, const float* in, int out_layout, void* out0, void* out1) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (!in || !out0 || (out_layout == ZLFFT_SOA && !out1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1;
+    const Buf bi{in, nullptr, 4 * n, 0};
+    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 8 * nb : 4 * nb,
+                 out_layout == ZLFFT_SOA ? 4 * nb : 0};
+    return run_sync(p, bi, bo, [&](const void* i0, const void*, void* o0, void* o1, hipStream_t s) {
+        return zlfft_rfft_forward_exec(p, static_cast<const float*>(i0), out_layout, o0, o1, 1, s);
+    });
+}
+
+int zlfft_rfft_forward_sqr_mag(zlfft_plan p, const float* in, float* out) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!in || !out) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1;
+    const Buf bi{in, nullptr, 4 * n, 0};
+    const Buf bo{out, nullptr, 4 * nb, 0};
+    return run_sync(p, bi, bo, [&](const void* i0, const void*, void* o0, void*, hipStream_t s) {
+        return zlfft_rfft_forward_sqr_mag_exec(p, static_cast<const float*>(i0), static_cast<float*>(o0), 1, s);
+    });
+}
+
+int zlfft_rfft_backward(zlfft_plan p, int in_layout, const void* in0, const void* in1, float* out) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!valid_layout(in_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (!in0 || !out || (in_layout == ZLFFT_SOA && !in1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1;
+    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 8 * nb : 4 * nb,
+                 in_layout == ZLFFT_SOA ? 4 * nb : 0};
+    const Buf bo{out, nullptr, 4 * n, 0};
+    return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void*, hipStream_t s) {
+        return zlfft_rfft_backward_exec(p, in_layout, i0, i1, static_cast<float*>(o0), 1, s);
+    });
+}
+
+int zlfft_plan_copy_rfft_twiddles(zlfft_plan p, float* host_out, size_t n_floats) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!host_out) return fail(ZLFFT_ERR_NULL, "host_out is NULL");
+    const size_t need = 2 * (size_t)p->n_rtw;
+    if (n_floats < need) return fail(ZLFFT_ERR_INVALID_ARG, "buffer too small");
+    if (need) ZL_HIP(hipMemcpy(host_out, p->d_rtw, need * sizeof(float), hipMemcpyDeviceToHost));
+    return ZLFFT_OK;
+}
+
+}  // extern "C"

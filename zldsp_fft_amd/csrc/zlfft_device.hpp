@@ -1,0 +1,402 @@
+// Device side of libzlfft: register codelets, LDS-exchanged Stockham passes
+// and the fused RFFT epilogue/prologue, for gfx950 (wave64).
+//
+// One kernel instantiation handles one CFFT order n (N = 2^n) for one kind
+// (C2C, real->complex, real->|X|^2, complex->real) and one pair of user
+// layouts.  Each transform is owned by T = N/R threads holding R points in
+// registers; TPW transforms share a workgroup.  Pass p of the Stockham
+// decomposition has radix r_p (the first pass takes the remainder radix) and
+// width w_p = r_0 ... r_{p-1}.  Thread t computes butterflies i = t + T b and
+// always holds register j = element t + T j between passes, so the first
+// pass reads and the last pass writes user memory in coalesced order; only
+// the N-point exchanges between passes go through LDS.
+//
+// Arithmetic conventions follow the reference (zldsp::fft):
+//   forward X[k] = sum_n x[n] e^{-2 pi i nk/N}, unnormalised
+//     (reference src/common/zldsp_fft_common_radix.hpp:149-156);
+//   backward = swap(re, im) -> forward -> swap(re, im)
+//     (reference src/common/zldsp_fft_common_structure.hpp:46-135);
+//   complex multiply re = fma(-ai, wi, ar*wr), im = fma(ai, wr, ar*wi)
+//     (reference src/common/zldsp_fft_common_radix.hpp:115-129).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace zl {
+
+enum Kind : int { C2C = 0, R2C = 1, R2C_SQR = 2, C2R = 3 };
+enum Layout : int { AOS = 0, SOA = 1 };
+
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Geometry of one LDS-resident transform.  LOGN: CFFT order; LOGR_: log2 of
+// points per thread; TPB: target workgroup size.
+template <int LOGN, int LOGR_, int TPB>
+struct Geo {
+    static constexpr int logn = LOGN;
+    static constexpr int LOGR = LOGN < LOGR_ ? LOGN : LOGR_;
+    static constexpr int N = 1 << LOGN;
+    static constexpr int R = 1 << LOGR;
+    static constexpr int T = N / R;
+    static constexpr int TPW = T >= TPB ? 1 : TPB / T;
+    static constexpr int WG = T * TPW;
+    static constexpr int NP = LOGN == 0 ? 0 : cdiv(LOGN, LOGR);
+    // log2 radix of pass p: the first pass takes the remainder
+    static constexpr int lr(int p) { return p == 0 ? LOGN - (NP - 1) * LOGR : LOGR; }
+    // log2 width of pass p
+    static constexpr int lw(int p) { return p == 0 ? 0 : lr(0) + (p - 1) * LOGR; }
+    // offset (in complex entries) of pass p's twiddle block; block p holds
+    // W_{r w}^{m k} at [(m - 1) w + k], m = 1..r-1, k = 0..w-1
+    static constexpr int tw_off(int p) {
+        int o = 0;
+        for (int q = 1; q < p; ++q) o += ((1 << lr(q)) - 1) << lw(q);
+        return o;
+    }
+    static constexpr int tw_total() { return tw_off(NP); }
+    // LDS row per transform: one pad slot per 16 complex values
+    static constexpr int LDSN = N + (N >> 4) + 1;
+    static constexpr int pad(int e) { return e + (e >> 4); }
+};
+
+// ---------------------------------------------------------------- codelets
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (-i)
+__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
+    return make_float2(__builtin_fmaf(-a.y, w.y, a.x * w.x), __builtin_fmaf(a.y, w.x, a.x * w.y));
+}
+__device__ __forceinline__ float2 swapri(float2 a) { return make_float2(a.y, a.x); }
+
+constexpr float kH = 0.70710678118654752440f;  // 1/sqrt(2)
+constexpr float kC16 = 0.92387953251128675613f; // cos(pi/8)
+constexpr float kS16 = 0.38268343236508977173f; // sin(pi/8)
+
+// x * W_16^e for the exponents a radix-16 codelet needs
+template <int E>
+__device__ __forceinline__ float2 w16(float2 a) {
+    constexpr int e = E & 15;
+    if constexpr (e == 0) return a;
+    else if constexpr (e == 4) return mul_mi(a);
+    else if constexpr (e == 8) return make_float2(-a.x, -a.y);
+    else if constexpr (e == 12) return make_float2(-a.y, a.x);
+    else if constexpr (e == 2) return make_float2((a.x + a.y) * kH, (a.y - a.x) * kH);
+    else if constexpr (e == 6) return make_float2((a.y - a.x) * kH, (-(a.x + a.y)) * kH);
+    else if constexpr (e == 1) return cmul(a, make_float2(kC16, -kS16));
+    else if constexpr (e == 3) return cmul(a, make_float2(kS16, -kC16));
+    else if constexpr (e == 9) return cmul(a, make_float2(-kC16, kS16));
+    else {
+        static_assert(e == 0, "unsupported W16 exponent");
+        return a;
+    }
+}
+
+// In-place forward DFT-r on x[0..r), natural-order output.
+template <int r>
+__device__ __forceinline__ void dft(float2* x);
+
+template <>
+__device__ __forceinline__ void dft<1>(float2*) {}
+
+template <>
+__device__ __forceinline__ void dft<2>(float2* x) {
+    const float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+}
+
+// radix-4 butterfly in the reference's operation order
+// (reference src/common/zldsp_fft_common_radix.hpp:184-235)
+template <>
+__device__ __forceinline__ void dft<4>(float2* x) {
+    const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
+    const float2 t2 = cadd(x[1], x[3]), t3 = csub(x[1], x[3]);
+    x[0] = cadd(t0, t2);
+    x[2] = csub(t0, t2);
+    x[1] = make_float2(t1.x + t3.y, t1.y - t3.x);
+    x[3] = make_float2(t1.x - t3.y, t1.y + t3.x);
+}
+
+// radix-8 as two radix-4 plus W_8 via 1/sqrt(2)
+// (reference src/common/zldsp_fft_common_radix.hpp:499-560)
+template <>
+__device__ __forceinline__ void dft<8>(float2* x) {
+    float2 e[4] = {x[0], x[2], x[4], x[6]};
+    float2 o[4] = {x[1], x[3], x[5], x[7]};
+    dft<4>(e);
+    dft<4>(o);
+    o[1] = w16<2>(o[1]);
+    o[2] = w16<4>(o[2]);
+    o[3] = w16<6>(o[3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        x[q] = cadd(e[q], o[q]);
+        x[q + 4] = csub(e[q], o[q]);
+    }
+}
+
+// radix-16 as 4 x 4: DFT-4 over x[j + 4m], twiddle W16^{j q}, DFT-4 over j.
+template <>
+__device__ __forceinline__ void dft<16>(float2* x) {
+    float2 b[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        b[j][0] = x[j];
+        b[j][1] = x[j + 4];
+        b[j][2] = x[j + 8];
+        b[j][3] = x[j + 12];
+        dft<4>(b[j]);
+    }
+    b[1][1] = w16<1>(b[1][1]);
+    b[1][2] = w16<2>(b[1][2]);
+    b[1][3] = w16<3>(b[1][3]);
+    b[2][1] = w16<2>(b[2][1]);
+    b[2][2] = w16<4>(b[2][2]);
+    b[2][3] = w16<6>(b[2][3]);
+    b[3][1] = w16<3>(b[3][1]);
+    b[3][2] = w16<6>(b[3][2]);
+    b[3][3] = w16<9>(b[3][3]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float2 c[4] = {b[0][q], b[1][q], b[2][q], b[3][q]};
+        dft<4>(c);
+        x[q] = c[0];
+        x[q + 4] = c[1];
+        x[q + 8] = c[2];
+        x[q + 12] = c[3];
+    }
+}
+
+// ---------------------------------------------------------------- passes
+
+// Pass P of geometry G on registers v; slds is this transform's LDS row.
+// Non-last passes leave the next pass's inputs in v (exchanged through LDS).
+template <class G, int P>
+__device__ __forceinline__ void run_pass(float2 (&v)[G::R], float2* slds, const float2* __restrict__ tw, int t) {
+    constexpr int lr = G::lr(P), r = 1 << lr, nb = G::R / r, lw = G::lw(P), w = 1 << lw;
+    constexpr bool last = (P == G::NP - 1);
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+        float2 x[r];
+#pragma unroll
+        for (int m = 0; m < r; ++m) x[m] = v[b + m * nb];
+        if constexpr (P > 0) {
+            const int i = t + G::T * b;
+            const int k = i & (w - 1);
+            const float2* twp = tw + G::tw_off(P) + k;
+#pragma unroll
+            for (int m = 1; m < r; ++m) x[m] = cmul(x[m], twp[(m - 1) * w]);
+        }
+        dft<r>(x);
+#pragma unroll
+        for (int q = 0; q < r; ++q) v[b + q * nb] = x[q];
+    }
+    if constexpr (!last) {
+        __syncthreads();  // every reader of the previous LDS contents is done
+#pragma unroll
+        for (int b = 0; b < nb; ++b) {
+            const int i = t + G::T * b;
+            const int base = ((i >> lw) << (lw + lr)) + (i & (w - 1));
+#pragma unroll
+            for (int q = 0; q < r; ++q) slds[G::pad(base + (q << lw))] = v[b + q * nb];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
+        run_pass<G, P + 1>(v, slds, tw, t);
+    }
+}
+
+template <class G>
+__device__ __forceinline__ void run_passes(float2 (&v)[G::R], float2* slds, const float2* __restrict__ tw, int t) {
+    if constexpr (G::NP > 0) run_pass<G, 0>(v, slds, tw, t);
+}
+
+// ---------------------------------------------------------------- kernel
+
+struct KArgs {
+    const void* in0;   // AoS: float2*, SoA: real part, real input: float*
+    const void* in1;   // SoA: imag part
+    void* out0;
+    void* out1;
+    const float2* tw;  // Stockham pass twiddles of this geometry
+    const float2* rtw; // RFFT (wc, ws) for k = 1..M/2, at [k - 1]
+    long long batch;
+    int backward;      // C2C only
+};
+
+__device__ __forceinline__ float2 ld_c(const void* p0, const void* p1, int lay, long long e) {
+    if (lay == AOS) return reinterpret_cast<const float2*>(p0)[e];
+    return make_float2(reinterpret_cast<const float*>(p0)[e], reinterpret_cast<const float*>(p1)[e]);
+}
+__device__ __forceinline__ void st_c(void* p0, void* p1, int lay, long long e, float2 v) {
+    if (lay == AOS) {
+        reinterpret_cast<float2*>(p0)[e] = v;
+    } else {
+        reinterpret_cast<float*>(p0)[e] = v.x;
+        reinterpret_cast<float*>(p1)[e] = v.y;
+    }
+}
+
+// Fused LDS-resident transform.
+//   C2C    : user layout LIN -> user layout LOUT, direction at run time
+//   R2C    : N real (= M complex, AoS view) -> M+1 bins in layout LOUT
+//            (reference RFFT::forward, src/zldsp_fft_rfft.hpp:68-79, 118-124)
+//   R2C_SQR: N real -> M+1 squared magnitudes
+//            (reference RFFT::forward_sqr_mag, src/zldsp_fft_rfft.hpp:104-109)
+//   C2R    : M+1 bins in layout LIN -> N real
+//            (reference RFFT::backward, src/zldsp_fft_rfft.hpp:86-97, 132-138)
+// For the real kinds G describes the inner CFFT of size M = N/2.
+template <class G, int KIND, int LIN, int LOUT>
+__global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
+    __shared__ float2 lds[G::TPW * G::LDSN];
+    const int tid = threadIdx.x;
+    const int slot = tid / G::T;
+    const int t = tid - slot * G::T;
+    const long long f = (long long)blockIdx.x * G::TPW + slot;
+    const bool active = f < a.batch;
+    float2* slds = lds + slot * G::LDSN;
+    constexpr int M = G::N;  // CFFT size
+    constexpr int ELEM_IN = (KIND == C2C && LIN == SOA) ? 4 : 8;
+    constexpr int ELEM_OUT = (KIND == C2C && LOUT == SOA) ? 4 : 8;
+    constexpr bool direct_in = G::T * ELEM_IN >= 128 || G::TPW == 1;
+    constexpr bool direct_out = G::T * ELEM_OUT >= 128 || G::TPW == 1;
+    const bool bwd = (KIND == C2C) ? (a.backward != 0) : (KIND == C2R);
+
+    float2 v[G::R];
+
+    // ------------------------------------------------------------ input
+    if constexpr (KIND == C2R) {
+        // pre-twiddle prologue (reference src/common/zldsp_fft_common_execute.hpp:398-468):
+        // Z written to LDS already swapped for the backward CFFT.
+        const long long fb = f * (M + 1);
+        if (active) {
+            if (t == 0) {
+                const float r0 = ld_c(a.in0, a.in1, LIN, fb).x;
+                const float rm = ld_c(a.in0, a.in1, LIN, fb + M).x;
+                slds[0] = swapri(make_float2((r0 + rm) * 0.5f, (r0 - rm) * 0.5f));
+            }
+            for (int k = 1 + t; k <= M / 2; k += G::T) {
+                const float2 x1 = ld_c(a.in0, a.in1, LIN, fb + k);
+                const float2 x2 = ld_c(a.in0, a.in1, LIN, fb + (M - k));
+                const float2 w = a.rtw[k - 1];  // (wc, ws)
+                const float si = x1.y + x2.y, dr = x1.x - x2.x;
+                const float zr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(si, w.x, x2.x));
+                const float zi_m = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x2.y));
+                if (k != M - k) {
+                    const float zr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(-si, w.x, x1.x));
+                    const float zi_k = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x1.y));
+                    slds[G::pad(k)] = make_float2(zi_k, zr_k);
+                }
+                slds[G::pad(M - k)] = make_float2(zi_m, zr_m);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
+    } else {
+        constexpr int lin = (KIND == C2C) ? LIN : AOS;  // real input viewed as M complex
+        if constexpr (direct_in) {
+            const long long fb = f * M;
+            if (active) {
+#pragma unroll
+                for (int j = 0; j < G::R; ++j) v[j] = ld_c(a.in0, a.in1, lin, fb + t + G::T * j);
+            } else {
+#pragma unroll
+                for (int j = 0; j < G::R; ++j) v[j] = make_float2(0.f, 0.f);
+            }
+        } else {
+            // coalesced cooperative copy of the workgroup's contiguous block
+            const long long wb = (long long)blockIdx.x * G::TPW * M;
+            const long long wn = (a.batch - (long long)blockIdx.x * G::TPW) * M;
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) {
+                const int e = tid + G::WG * j;
+                const int s = e / M, pos = e - s * M;
+                lds[s * G::LDSN + G::pad(pos)] = (e < wn) ? ld_c(a.in0, a.in1, lin, wb + e) : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
+        }
+        if (bwd) {
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) v[j] = swapri(v[j]);
+        }
+    }
+
+    // ------------------------------------------------------------ passes
+    run_passes<G>(v, slds, a.tw, t);
+
+    // ------------------------------------------------------------ output
+    if constexpr (KIND == R2C || KIND == R2C_SQR) {
+        // post-twiddle epilogue (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552)
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
+        __syncthreads();
+        if (active) {
+            const long long fb = f * (M + 1);
+            if (t == 0) {
+                const float2 z0 = slds[0];
+                const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+                if constexpr (KIND == R2C) {
+                    st_c(a.out0, a.out1, LOUT, fb, make_float2(dc, 0.f));
+                    st_c(a.out0, a.out1, LOUT, fb + M, make_float2(ny, 0.f));
+                } else {
+                    float* o = reinterpret_cast<float*>(a.out0);
+                    o[fb] = dc * dc;
+                    o[fb + M] = ny * ny;
+                }
+            }
+            for (int k = 1 + t; k <= M / 2; k += G::T) {
+                const float2 z1 = slds[G::pad(k)];
+                const float2 z2 = slds[G::pad(M - k)];
+                const float2 w = a.rtw[k - 1];
+                const float si = z1.y + z2.y, dr = z1.x - z2.x;
+                const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
+                const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
+                const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
+                const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
+                if constexpr (KIND == R2C) {
+                    if (k != M - k) st_c(a.out0, a.out1, LOUT, fb + k, make_float2(xr_k, xi_k));
+                    st_c(a.out0, a.out1, LOUT, fb + (M - k), make_float2(xr_m, xi_m));
+                } else {
+                    float* o = reinterpret_cast<float*>(a.out0);
+                    if (k != M - k) o[fb + k] = __builtin_fmaf(xr_k, xr_k, xi_k * xi_k);
+                    o[fb + (M - k)] = __builtin_fmaf(xr_m, xr_m, xi_m * xi_m);
+                }
+            }
+        }
+    } else {
+        constexpr int lout = (KIND == C2C) ? LOUT : AOS;  // real output viewed as M complex
+        if (bwd) {
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) v[j] = swapri(v[j]);
+        }
+        if constexpr (direct_out) {
+            if (active) {
+                const long long fb = f * M;
+#pragma unroll
+                for (int j = 0; j < G::R; ++j) st_c(a.out0, a.out1, lout, fb + t + G::T * j, v[j]);
+            }
+        } else {
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
+            __syncthreads();
+            const long long wb = (long long)blockIdx.x * G::TPW * M;
+            const long long wn = (a.batch - (long long)blockIdx.x * G::TPW) * M;
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) {
+                const int e = tid + G::WG * j;
+                const int s = e / M, pos = e - s * M;
+                if (e < wn) st_c(a.out0, a.out1, lout, wb + e, lds[s * G::LDSN + G::pad(pos)]);
+            }
+        }
+    }
+}
+
+}  // namespace zl

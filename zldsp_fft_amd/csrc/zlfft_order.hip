@@ -1,0 +1,25 @@
+// One CFFT order's LDS kernels (compiled once per order with -DZL_LOGN=n so
+// the build runs in parallel): C2C for the four AoS/SoA combinations and the
+// real kinds, whose inner CFFT has this order.
+#include "zlfft_registry.hpp"
+
+#ifndef ZL_LOGN
+#error "compile with -DZL_LOGN=<order>"
+#endif
+
+#define ZL_CAT2(a, b) a##b
+#define ZL_CAT(a, b) ZL_CAT2(a, b)
+
+void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
+    using namespace zl;
+    using G = Geo<ZL_LOGN, cfg_logr(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
+    tab[C2C][AOS][AOS] = make_entry<G, C2C, AOS, AOS>();
+    tab[C2C][AOS][SOA] = make_entry<G, C2C, AOS, SOA>();
+    tab[C2C][SOA][AOS] = make_entry<G, C2C, SOA, AOS>();
+    tab[C2C][SOA][SOA] = make_entry<G, C2C, SOA, SOA>();
+    tab[R2C][AOS][AOS] = make_entry<G, R2C, AOS, AOS>();
+    tab[R2C][AOS][SOA] = make_entry<G, R2C, AOS, SOA>();
+    tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS>();
+    tab[C2R][AOS][AOS] = make_entry<G, C2R, AOS, AOS>();
+    tab[C2R][SOA][AOS] = make_entry<G, C2R, SOA, AOS>();
+}
