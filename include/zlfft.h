@@ -23,6 +23,9 @@
  * asynchronously on `stream`, and process `batch` transforms stored back to
  * back (CFFT: N complex per transform; RFFT real side: N floats; RFFT complex
  * side: N/2+1 complex; squared magnitude: N/2+1 floats).
+ * CFFT orders up to 14 (RFFT up to 15) run as one LDS-resident kernel; larger
+ * ones run as 2-3 HBM passes through a plan-owned workspace that the first
+ * call of a given batch size allocates (hipMalloc) and later calls reuse.
  * Synchronous entry points (no `_exec` suffix) process one transform, accept
  * host or device pointers, and return when the result is in `out`; they are
  * what the drop-in C++ classes in zldsp_fft_cfft.hpp / zldsp_fft_rfft.hpp call.
@@ -68,8 +71,8 @@ extern "C" {
 #define ZLFFT_FORWARD 0
 #define ZLFFT_BACKWARD 1
 
-/* largest supported order: CFFT 2^ZLFFT_MAX_CFFT_ORDER points */
-#define ZLFFT_MAX_CFFT_ORDER 14
+/* largest supported order: CFFT 2^ZLFFT_MAX_CFFT_ORDER points (RFFT one more) */
+#define ZLFFT_MAX_CFFT_ORDER 28
 
 typedef struct zlfft_plan_s *zlfft_plan;
 

@@ -49,8 +49,8 @@ def _create(kind, prec, order):
 
 @pytest.mark.parametrize("kind,prec,order,code", [
     (zf.KIND_RFFT, 0, 0, 1),           # RFFT order 0: the reference underflows rfft_order - 1
-    (zf.KIND_CFFT, 0, 40, 1),          # beyond the supported maximum
-    (zf.KIND_RFFT, 0, 40, 1),
+    (zf.KIND_CFFT, 0, 29, 1),          # beyond the supported maximum
+    (zf.KIND_RFFT, 0, 30, 1),
     (7, 0, 10, 5),                     # unknown kind
     (zf.KIND_CFFT, 9, 10, 6),          # unknown precision
 ])

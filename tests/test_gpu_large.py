@@ -1,0 +1,87 @@
+"""Multi-pass (HBM) transforms, CFFT orders above 14 and RFFT orders above 15,
+against the oracle.  Tolerance rel-L2 <= 1e-6 per transform vs FP64."""
+import numpy as np
+import pytest
+
+import oracle
+import zldsp_fft_amd as zf
+from zldsp_fft_amd import signals
+from test_gpu_parity import LAYOUTS, fetch, make_in, make_out, rel, to_dev_complex
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+@pytest.mark.parametrize("order", range(15, 23))
+def test_cfft_large_orders(order):
+    n = 1 << order
+    batch = 2 if order <= 19 else 1
+    x = signals.complex_uniform(4000 + order, batch, n).astype(np.complex64).astype(np.complex128)
+    plan = zf.CFFT(order)
+    for forward in (True, False):
+        ref = np.stack([oracle.cfft(r, forward, "f64") for r in x])
+        first = None
+        for lin, lout in (LAYOUTS if order in (15, 16, 21, 22) else LAYOUTS[::3]):
+            yo = make_out((batch, n), lout)
+            (plan.forward if forward else plan.backward)(make_in(x, lin), yo)
+            y = fetch(yo)
+            e = rel(y, ref).max()
+            print(f"order={order} fwd={forward} lin={lin} lout={lout} relL2={e:.3e}", flush=True)
+            assert e <= TOL
+            if first is None:
+                first = y
+            else:
+                assert np.array_equal(y, first)
+
+
+@pytest.mark.parametrize("order,batch", [(16, 64), (18, 16), (20, 4), (22, 1)])
+def test_large_soa_config(order, batch):
+    """SoA CFFTs, re/im ~ N(0, 1), 4,194,304 points per call: forward vs the FP64
+    oracle on sampled transforms, backward(forward(x)) = N x on all."""
+    n = 1 << order
+    g = torch.Generator(device="cuda").manual_seed(3)
+    re = torch.randn((batch, n), device="cuda", generator=g)
+    im = torch.randn((batch, n), device="cuda", generator=g)
+    plan = zf.CFFT(order)
+    yr, yi = torch.empty_like(re), torch.empty_like(im)
+    zr, zi = torch.empty_like(re), torch.empty_like(im)
+    plan.forward((re, im), (yr, yi))
+    plan.backward((yr, yi), (zr, zi))
+    torch.cuda.synchronize()
+    num = torch.sqrt(torch.sum((zr - n * re) ** 2 + (zi - n * im) ** 2, dim=1))
+    den = torch.sqrt(torch.sum((n * re) ** 2 + (n * im) ** 2, dim=1))
+    rt = (num / den).max().item()
+    idx = sorted({0, batch - 1})
+    x = (re[idx].cpu().numpy().astype(np.float64) + 1j * im[idx].cpu().numpy().astype(np.float64))
+    y = (yr[idx].cpu().numpy().astype(np.float64) + 1j * yi[idx].cpu().numpy().astype(np.float64))
+    ref = np.stack([oracle.cfft(r, True, "f64") for r in x])
+    e = rel(y, ref).max()
+    print(f"order={order} batch={batch}: fwd relL2={e:.3e} round trip relL2={rt:.3e}", flush=True)
+    assert e <= TOL and rt <= TOL
+
+
+@pytest.mark.parametrize("order", [16, 17, 19])
+def test_rfft_large_orders(order):
+    n, nb = 1 << order, (1 << order) // 2 + 1
+    x = signals.uniform(5000 + order, n).astype(np.float32).astype(np.float64)
+    plan = zf.RFFT(order)
+    xd = torch.from_numpy(x.astype(np.float32)).cuda()
+    ref = oracle.rfft_forward(x, "f64")
+    for lay in (zf.AOS, zf.SOA):
+        y = make_out((nb,), lay)
+        plan.forward(xd, y)
+        Y = fetch(y)
+        assert rel(Y, ref) <= TOL and Y[0].imag == 0.0 and Y[-1].imag == 0.0
+    sq = torch.empty((nb,), dtype=torch.float32, device="cuda")
+    plan.forward_sqr_mag(xd, sq)
+    torch.cuda.synchronize()
+    s = sq.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(s - np.abs(ref) ** 2) / np.linalg.norm(np.abs(ref) ** 2) <= 2 * TOL
+    bins = signals.complex_uniform(6000 + order, 1, nb)[0].astype(np.complex64).astype(np.complex128)
+    want = oracle.rfft_backward(bins, n, "f64")
+    for lay in (zf.AOS, zf.SOA):
+        r = torch.empty((n,), dtype=torch.float32, device="cuda")
+        plan.backward(make_in(bins[None, :], lay) if lay == zf.SOA else to_dev_complex(bins), r)
+        torch.cuda.synchronize()
+        assert rel(r.cpu().numpy().astype(np.float64), want) <= TOL

@@ -54,7 +54,7 @@ def oracle_rows(x, forward, precision):
     return np.stack([oracle.cfft(r, forward, precision) for r in x])
 
 
-@pytest.mark.parametrize("order", range(0, zf.MAX_CFFT_ORDER + 1))
+@pytest.mark.parametrize("order", range(0, zf.MAX_LDS_ORDER + 1))
 def test_cfft_orders_layouts_directions(order):
     n = 1 << order
     batch = 3  # odd: exercises a partly filled workgroup
@@ -113,7 +113,7 @@ def test_round_trip_is_n_times_x(order):
     assert rel(fetch(z), n * xr).max() <= TOL
 
 
-@pytest.mark.parametrize("order", range(1, zf.MAX_CFFT_ORDER + 2))
+@pytest.mark.parametrize("order", range(1, zf.MAX_LDS_ORDER + 2))
 def test_rfft_orders(order):
     n, nb, batch = 1 << order, (1 << order) // 2 + 1, 3
     x = signals.uniform(2000 + order, batch * n).reshape(batch, n).astype(np.float32).astype(np.float64)

@@ -39,7 +39,8 @@ KIND_CFFT, KIND_RFFT = 0, 1
 PREC_F32 = 0
 AOS, SOA = 0, 1
 FORWARD, BACKWARD = 0, 1
-MAX_CFFT_ORDER = 14
+MAX_CFFT_ORDER = 28
+MAX_LDS_ORDER = 14
 
 _ERRNAMES = {1: "INVALID_ORDER", 2: "NULL", 3: "ALLOC", 4: "HIP", 5: "INVALID_ARG", 6: "UNSUPPORTED"}
 

@@ -19,6 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libzlfft.so")
 MAX_LDS_ORDER = 14
+MIN_MACRO_LOGL, MAX_MACRO_LOGL = 6, 10
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 HIP_FLAGS = [
@@ -47,6 +48,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     units = []
     for n in range(MAX_LDS_ORDER + 1):
         units.append((os.path.join(CSRC, "zlfft_order.hip"), os.path.join(OBJ, f"order{n}.o"), [f"-DZL_LOGN={n}"]))
+    for l in range(MIN_MACRO_LOGL, MAX_MACRO_LOGL + 1):
+        units.append((os.path.join(CSRC, "zlfft_macro.hip"), os.path.join(OBJ, f"macro{l}.o"), [f"-DZL_LOGL={l}"]))
     units.append((os.path.join(CSRC, "zlfft_api.hip"), os.path.join(OBJ, "api.o"), []))
     cmds = [[HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -64,7 +67,25 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         print(err, file=sys.stderr)
         raise RuntimeError("link of libzlfft.so failed")
     os.replace(tmp, LIB)
+    build_dropin_example()
     return LIB
+
+
+DROPIN_SRC = os.path.join(ROOT, "tests", "dropin", "dropin_example.cpp")
+DROPIN_BIN = os.path.join(ROOT, "tests", "dropin", "_build", "dropin_example")
+
+
+def build_dropin_example() -> str:
+    """README-style C++ program against the drop-in headers, linked to libzlfft.so."""
+    os.makedirs(os.path.dirname(DROPIN_BIN), exist_ok=True)
+    cxx = shutil.which("g++") or "c++"
+    cmd = [cxx, "-std=c++20", "-O2", "-Wall", "-Wextra", "-I", INCLUDE, DROPIN_SRC, "-o", DROPIN_BIN,
+           "-L", HERE, "-lzlfft", "-Wl,-rpath,$ORIGIN/../../../zldsp_fft_amd"]
+    _, rc, err = _run(cmd)
+    if rc:
+        print(err, file=sys.stderr)
+        raise RuntimeError("drop-in example build failed")
+    return DROPIN_BIN
 
 
 if __name__ == "__main__":

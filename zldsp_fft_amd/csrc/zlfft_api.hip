@@ -31,6 +31,7 @@ int hip_fail(hipError_t e, const char* what) {
     } while (0)
 
 zl::OrderTable g_tab[zl::kMaxLdsOrder + 1];
+zl::MacroTable g_macro[zl::kMaxMacroLogL + 1];
 std::once_flag g_tab_once;
 
 void fill_tables() {
@@ -49,6 +50,11 @@ void fill_tables() {
     zl_fill_order_12(g_tab[12]);
     zl_fill_order_13(g_tab[13]);
     zl_fill_order_14(g_tab[14]);
+    zl_fill_macro_6(g_macro[6]);
+    zl_fill_macro_7(g_macro[7]);
+    zl_fill_macro_8(g_macro[8]);
+    zl_fill_macro_9(g_macro[9]);
+    zl_fill_macro_10(g_macro[10]);
 }
 
 // Stockham twiddles of one pass: out[(m - 1) w + k] = W_{r w}^{m k}, computed
@@ -63,6 +69,18 @@ __global__ void gen_pass_twiddles(float2* out, int lr, int lw, int count) {
     const long long e = (m * k) & (rw - 1);
     double s, c;
     sincospi(-2.0 * (double)e / (double)rw, &s, &c);
+    out[idx] = make_float2((float)c, (float)s);
+}
+
+// Macro twiddle table of an HBM pass: out[idx] = W_{2^log_lw}^{idx << shift},
+// FP64 on the GPU, rounded to FP32.
+__global__ void gen_macro_twiddles(float2* out, int log_lw, int shift, int count) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= count) return;
+    const long long mod = 1ll << log_lw;
+    const long long e = ((long long)idx << shift) & (mod - 1);
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)mod, &s, &c);
     out[idx] = make_float2((float)c, (float)s);
 }
 
@@ -85,10 +103,19 @@ struct zlfft_plan_s {
     int order = 0;   // user order
     int corder = 0;  // inner CFFT order
     int device = 0;
-    const zl::OrderTable* tab = nullptr;
+    const zl::OrderTable* tab = nullptr;  // LDS kernels (corder <= kMaxLdsOrder)
     float2* d_tw = nullptr;
     float2* d_rtw = nullptr;
     int n_rtw = 0;
+    // multi-pass transforms (corder > kMaxLdsOrder)
+    bool macro = false;
+    int mp = 0;                 // HBM passes
+    int logl[4] = {}, logw[4] = {}, logh[4] = {};
+    float2* d_itw[4] = {};      // inner Stockham twiddles per pass
+    float2* d_twa[4] = {};      // macro twiddle tables per pass (p >= 1)
+    float2* d_twb[4] = {};
+    void* d_work = nullptr;     // intermediate buffers, grown on demand
+    size_t work_bytes = 0;
     // synchronous calls on host pointers: staging buffers and a stream
     void* d_stage = nullptr;
     size_t stage_bytes = 0;
@@ -102,8 +129,106 @@ int check_plan(zlfft_plan p) {
     return ZLFFT_OK;
 }
 
+int ensure_work(zlfft_plan p, size_t bytes) {
+    if (p->work_bytes >= bytes) return ZLFFT_OK;
+    if (p->d_work) {
+        // the previous workspace may still be in use by queued work on any stream
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+        (void)hipFree(p->d_work);
+        p->d_work = nullptr;
+        p->work_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&p->d_work, bytes);
+    if (e != hipSuccess) return fail(ZLFFT_ERR_ALLOC, std::string("workspace hipMalloc: ") + hipGetErrorString(e));
+    p->work_bytes = bytes;
+    return ZLFFT_OK;
+}
+
+// Multi-pass CFFT of order p->corder on `batch` transforms: pass 0 reads the
+// user layout lin, the last pass writes lout, intermediates ping-pong in work.
+int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void* in1, int lout, void* out0,
+               void* out1, size_t batch, void* stream, float2* work) {
+    const long long n = 1ll << p->corder;
+    float2* tmp[2] = {work, work + (size_t)batch * (size_t)n};
+    const void* s0 = in0;
+    const void* s1 = in1;
+    for (int q = 0; q < p->mp; ++q) {
+        const int role = q == 0 ? 0 : (q == p->mp - 1 ? 1 : 2);
+        const int lay = role == 0 ? lin : (role == 1 ? lout : zl::AOS);
+        const zl::MacroEntry& e = g_macro[p->logl[q]][role][lay];
+        if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no HBM-pass kernel for this layout");
+        void* d0 = role == 1 ? out0 : tmp[q & 1];
+        void* d1 = role == 1 ? out1 : nullptr;
+        zl::MArgs a{};
+        a.in0 = s0;
+        a.in1 = s1;
+        a.out0 = d0;
+        a.out1 = d1;
+        a.tw = p->d_itw[q];
+        a.twa = p->d_twa[q];
+        a.twb = p->d_twb[q];
+        a.batch = (long long)batch;
+        a.logn = p->corder;
+        a.logw = p->logw[q];
+        a.logh = p->logh[q];
+        a.backward = backward;
+        const size_t grid = batch * (size_t)(n >> p->logl[q]) / (size_t)e.cols;
+        if (grid > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
+        void* params[] = {&a};
+        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)grid), dim3(e.wg), params, 0,
+                               reinterpret_cast<hipStream_t>(stream)));
+        s0 = d0;
+        s1 = d1;
+    }
+    return ZLFFT_OK;
+}
+
+size_t macro_work_bytes(zlfft_plan p, size_t batch) {
+    const size_t n = (size_t)1 << p->corder;
+    const size_t bufs = p->mp > 2 ? 2 : 1;
+    size_t b = bufs * batch * n * sizeof(float2);
+    if (p->kind == ZLFFT_KIND_RFFT) b += batch * n * sizeof(float2);  // Z
+    return b;
+}
+
+int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
+                 size_t batch, int backward, void* stream) {
+    if (batch == 0) return ZLFFT_OK;
+    if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
+    float2* work = static_cast<float2*>(p->d_work);
+    if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work);
+    const size_t n = (size_t)1 << p->corder;
+    float2* z = work + (p->mp > 2 ? 2 : 1) * batch * n;
+    const long long threads = (long long)batch * (long long)(n / 2 + 1);
+    const unsigned grid = (unsigned)((threads + 255) / 256);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (kind == zl::R2C || kind == zl::R2C_SQR) {
+        if (int rc = macro_cfft(p, 0, zl::AOS, in0, nullptr, zl::AOS, z, nullptr, batch, stream, work)) return rc;
+        if (kind == zl::R2C_SQR)
+            zl::rfft_post_kernel<zl::R2C_SQR, zl::AOS><<<grid, 256, 0, s>>>(z, out0, nullptr, p->d_rtw, p->corder,
+                                                                            (long long)batch);
+        else if (lout == zl::AOS)
+            zl::rfft_post_kernel<zl::R2C, zl::AOS><<<grid, 256, 0, s>>>(z, out0, out1, p->d_rtw, p->corder,
+                                                                        (long long)batch);
+        else
+            zl::rfft_post_kernel<zl::R2C, zl::SOA><<<grid, 256, 0, s>>>(z, out0, out1, p->d_rtw, p->corder,
+                                                                        (long long)batch);
+        ZL_HIP(hipGetLastError());
+        return ZLFFT_OK;
+    }
+    // C2R
+    if (lin == zl::AOS)
+        zl::rfft_pre_kernel<zl::AOS><<<grid, 256, 0, s>>>(in0, in1, z, p->d_rtw, p->corder, (long long)batch);
+    else
+        zl::rfft_pre_kernel<zl::SOA><<<grid, 256, 0, s>>>(in0, in1, z, p->d_rtw, p->corder, (long long)batch);
+    ZL_HIP(hipGetLastError());
+    return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work);
+}
+
 int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
            size_t batch, int backward, void* stream) {
+    if (p->macro) return launch_macro(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream);
     const zl::KernelEntry& e = (*p->tab)[kind][lin][lout];
     if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no kernel for this kind/layout combination");
     if (batch == 0) return ZLFFT_OK;
@@ -204,7 +329,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     if (kind == ZLFFT_KIND_RFFT && order < 1)
         return fail(ZLFFT_ERR_INVALID_ORDER, "RFFT order must be >= 1 (the reference underflows at 0)");
     const size_t corder = kind == ZLFFT_KIND_RFFT ? order - 1 : order;
-    if (corder > (size_t)zl::kMaxLdsOrder)
+    if (corder > (size_t)zl::kMaxOrder)
         return fail(ZLFFT_ERR_INVALID_ORDER, "order " + std::to_string(order) + " exceeds the supported maximum");
     std::call_once(g_tab_once, fill_tables);
 
@@ -214,7 +339,8 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     p->precision = precision;
     p->order = (int)order;
     p->corder = (int)corder;
-    p->tab = &g_tab[corder];
+    p->macro = corder > (size_t)zl::kMaxLdsOrder;
+    p->tab = p->macro ? nullptr : &g_tab[corder];
     auto cleanup = [&](int rc) {
         zlfft_plan_destroy(p);
         return rc;
@@ -224,7 +350,42 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreateWithFlags"));
 
-    const zl::KernelEntry& ent = (*p->tab)[zl::C2C][0][0];
+    if (p->macro) {
+        const zl::MacroPlan mpl = zl::macro_plan((int)corder);
+        p->mp = mpl.np;
+        int lw = 0;
+        for (int q = 0; q < mpl.np; ++q) {
+            p->logl[q] = mpl.logl[q];
+            p->logw[q] = lw;
+            const zl::MacroEntry& me = g_macro[mpl.logl[q]][0][0];
+            if (me.tw_total > 0) {
+                e = hipMalloc(&p->d_itw[q], sizeof(float2) * (size_t)me.tw_total);
+                if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
+                size_t off = 0;
+                for (int r = 1; r < me.np; ++r) {
+                    const int cnt = ((1 << me.lr[r]) - 1) << me.lw[r];
+                    gen_pass_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_itw[q] + off, me.lr[r],
+                                                                                  me.lw[r], cnt);
+                    off += (size_t)cnt;
+                }
+            }
+            if (q > 0) {
+                const int log_lw = lw + mpl.logl[q];
+                const int h = (log_lw + 1) / 2;
+                p->logh[q] = h;
+                const int na = 1 << (log_lw - h), nb = 1 << h;
+                e = hipMalloc(&p->d_twa[q], sizeof(float2) * (size_t)na);
+                if (e == hipSuccess) e = hipMalloc(&p->d_twb[q], sizeof(float2) * (size_t)nb);
+                if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
+                gen_macro_twiddles<<<(na + 255) / 256, 256, 0, p->stream>>>(p->d_twa[q], log_lw, h, na);
+                gen_macro_twiddles<<<(nb + 255) / 256, 256, 0, p->stream>>>(p->d_twb[q], log_lw, 0, nb);
+            }
+            lw += mpl.logl[q];
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
+    }
+    const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[zl::C2C][0][0];
     if (ent.tw_total > 0) {
         e = hipMalloc(&p->d_tw, sizeof(float2) * (size_t)ent.tw_total);
         if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
@@ -260,6 +421,12 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (p->d_tw) (void)hipFree(p->d_tw);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
     if (p->d_stage) (void)hipFree(p->d_stage);
+    if (p->d_work) (void)hipFree(p->d_work);
+    for (int q = 0; q < 4; ++q) {
+        if (p->d_itw[q]) (void)hipFree(p->d_itw[q]);
+        if (p->d_twa[q]) (void)hipFree(p->d_twa[q]);
+        if (p->d_twb[q]) (void)hipFree(p->d_twb[q]);
+    }
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
     return ZLFFT_OK;

@@ -399,4 +399,174 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
     }
 }
 
+// ---------------------------------------------------------------- large N
+
+// Arguments of one HBM pass of a large transform.
+struct MArgs {
+    const void* in0;
+    const void* in1;
+    void* out0;
+    void* out1;
+    const float2* tw;   // inner Stockham twiddles of the radix-L sub-FFT (geometry GL)
+    const float2* twa;  // macro twiddle table A[a] = W_{L w}^{a 2^h}
+    const float2* twb;  // macro twiddle table B[b] = W_{L w}^{b}, b < 2^h
+    long long batch;
+    int logn;           // log2 N of the whole transform
+    int logw;           // log2 width w of this pass (product of earlier radices)
+    int logh;           // split of the two-level macro twiddle
+    int backward;       // swap re/im at the user-side load (first pass) and store (last pass)
+};
+
+// One pass of a multi-pass Stockham DIT transform of N = 2^logn points:
+// radix L = GL::N, width w, N/L butterflies i = (j, k), k = i mod w.
+// Butterfly i reads x[i + m N/L] (m < L), multiplies by W_{L w}^{m k}, takes a
+// length-L DFT in LDS and writes y_q to out[j L w + k + q w].  A workgroup owns
+// C consecutive butterflies of one transform ("columns"): lanes run over
+// columns first so every global access covers C consecutive elements.
+// FIRST: reads the user layout LIN (w = 1, no macro twiddle) and writes its
+// contiguous output block through LDS; LAST: writes the user layout LOUT.
+// Intermediate buffers are AoS float2.
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT>
+__global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
+    __shared__ float2 lds[C * GL::LDSN];
+    constexpr int L = GL::N, T = GL::T, R = GL::R, WG = C * T;
+    const int tid = threadIdx.x;
+    const int c = tid % C;
+    const int t = tid / C;
+    const long long n = 1ll << a.logn;
+    const long long cols = n >> GL::logn;  // butterflies per transform
+    const long long cblocks = cols / C;
+    const long long b = (long long)blockIdx.x / cblocks;
+    const long long i0 = ((long long)blockIdx.x - b * cblocks) * C;
+    const long long i = i0 + c;
+    const long long base = b * n;
+    constexpr int lin = FIRST ? LIN : AOS;
+    constexpr int lout = LAST ? LOUT : AOS;
+    float2* slds = lds + c * GL::LDSN;
+
+    float2 v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = ld_c(a.in0, a.in1, lin, base + i + (long long)(t + T * j) * cols);
+    if (FIRST && a.backward) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
+    }
+    if constexpr (!FIRST) {
+        // two-level macro twiddle W_{L w}^{m k}: e = m k mod (L w), W^e = A[e >> h] B[e & (2^h - 1)]
+        const long long k = i & ((1ll << a.logw) - 1);
+        const long long emask = (1ll << (a.logw + GL::logn)) - 1;
+        const long long hmask = (1ll << a.logh) - 1;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const long long m = t + T * j;
+            const long long e = (m * k) & emask;
+            const float2 w = cmul(a.twa[e >> a.logh], a.twb[e & hmask]);
+            v[j] = cmul(v[j], w);
+        }
+    }
+    run_passes<GL>(v, slds, a.tw, t);  // v[j] = y_q with q = t + T j
+    if (LAST && a.backward) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
+    }
+    if constexpr (FIRST) {
+        // w = 1: out[i L + q]; the workgroup's output block [i0 L, (i0 + C) L) is contiguous
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < R; ++j) slds[GL::pad(t + T * j)] = v[j];
+        __syncthreads();
+        const long long ob = base + i0 * L;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int e = tid + WG * j;
+            const int cc = e / L, q = e - cc * L;
+            st_c(a.out0, a.out1, lout, ob + e, lds[cc * GL::LDSN + GL::pad(q)]);
+        }
+    } else {
+        const long long w = 1ll << a.logw;
+        const long long ob = base + ((i >> a.logw) << (a.logw + GL::logn)) + (i & (w - 1));
+#pragma unroll
+        for (int j = 0; j < R; ++j) st_c(a.out0, a.out1, lout, ob + (long long)(t + T * j) * w, v[j]);
+    }
+}
+
+// RFFT post-twiddle / squared magnitude over a CFFT result Z (AoS, M complex
+// per frame) for sizes whose inner CFFT runs as several HBM passes.  One
+// thread per pair (k, M - k), k = 0..M/2; k = 0 writes DC and Nyquist.
+// Same formulas as the LDS kernel's fused epilogue
+// (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552).
+template <int KIND, int LOUT>
+__global__ void __launch_bounds__(256) rfft_post_kernel(const float2* __restrict__ z, void* out0, void* out1,
+                                                        const float2* __restrict__ rtw, int logm, long long batch) {
+    const long long m = 1ll << logm;
+    const long long half = m >> 1;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long f = g / (half + 1);
+    const long long k = g - f * (half + 1);
+    if (f >= batch) return;
+    const float2* zf = z + f * m;
+    const long long fb = f * (m + 1);
+    if (k == 0) {
+        const float2 z0 = zf[0];
+        const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+        if constexpr (KIND == R2C) {
+            st_c(out0, out1, LOUT, fb, make_float2(dc, 0.f));
+            st_c(out0, out1, LOUT, fb + m, make_float2(ny, 0.f));
+        } else {
+            reinterpret_cast<float*>(out0)[fb] = dc * dc;
+            reinterpret_cast<float*>(out0)[fb + m] = ny * ny;
+        }
+        return;
+    }
+    const float2 z1 = zf[k], z2 = zf[m - k];
+    const float2 w = rtw[k - 1];
+    const float si = z1.y + z2.y, dr = z1.x - z2.x;
+    const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
+    const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
+    const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
+    const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
+    if constexpr (KIND == R2C) {
+        if (k != m - k) st_c(out0, out1, LOUT, fb + k, make_float2(xr_k, xi_k));
+        st_c(out0, out1, LOUT, fb + (m - k), make_float2(xr_m, xi_m));
+    } else {
+        float* o = reinterpret_cast<float*>(out0);
+        if (k != m - k) o[fb + k] = __builtin_fmaf(xr_k, xr_k, xi_k * xi_k);
+        o[fb + (m - k)] = __builtin_fmaf(xr_m, xr_m, xi_m * xi_m);
+    }
+}
+
+// RFFT pre-twiddle for the multi-pass sizes: bins (layout LIN, M+1 per frame)
+// -> Z (AoS, M per frame), the input of the backward CFFT
+// (reference src/common/zldsp_fft_common_execute.hpp:398-468).
+template <int LIN>
+__global__ void __launch_bounds__(256) rfft_pre_kernel(const void* in0, const void* in1, float2* __restrict__ z,
+                                                       const float2* __restrict__ rtw, int logm, long long batch) {
+    const long long m = 1ll << logm;
+    const long long half = m >> 1;
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long f = g / (half + 1);
+    const long long k = g - f * (half + 1);
+    if (f >= batch) return;
+    float2* zf = z + f * m;
+    const long long fb = f * (m + 1);
+    if (k == 0) {
+        const float r0 = ld_c(in0, in1, LIN, fb).x;
+        const float rm = ld_c(in0, in1, LIN, fb + m).x;
+        zf[0] = make_float2((r0 + rm) * 0.5f, (r0 - rm) * 0.5f);
+        return;
+    }
+    const float2 x1 = ld_c(in0, in1, LIN, fb + k);
+    const float2 x2 = ld_c(in0, in1, LIN, fb + (m - k));
+    const float2 w = rtw[k - 1];
+    const float si = x1.y + x2.y, dr = x1.x - x2.x;
+    if (k != m - k) {
+        const float zr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(-si, w.x, x1.x));
+        const float zi_k = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x1.y));
+        zf[k] = make_float2(zr_k, zi_k);
+    }
+    const float zr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(si, w.x, x2.x));
+    const float zi_m = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x2.y));
+    zf[m - k] = make_float2(zr_m, zi_m);
+}
+
 }  // namespace zl

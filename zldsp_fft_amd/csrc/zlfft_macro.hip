@@ -1,0 +1,21 @@
+// HBM-pass kernels of radix 2^ZL_LOGL for the multi-pass large transforms
+// (compiled once per radix with -DZL_LOGL=l).
+#include "zlfft_registry.hpp"
+
+#ifndef ZL_LOGL
+#error "compile with -DZL_LOGL=<log2 radix>"
+#endif
+
+#define ZL_CAT2(a, b) a##b
+#define ZL_CAT(a, b) ZL_CAT2(a, b)
+
+void ZL_CAT(zl_fill_macro_, ZL_LOGL)(zl::MacroTable& tab) {
+    using namespace zl;
+    using GL = Geo<ZL_LOGL, 4, 1>;
+    constexpr int C = cfg_macro_c(ZL_LOGL);
+    tab[0][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS>();
+    tab[0][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS>();
+    tab[1][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS>();
+    tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA>();
+    tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS>();
+}

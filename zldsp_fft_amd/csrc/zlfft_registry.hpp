@@ -42,7 +42,80 @@ KernelEntry make_entry() {
     return e;
 }
 
+// ---- multi-pass (HBM) transforms for orders above kMaxLdsOrder
+
+constexpr int kMinMacroLogL = 6;
+constexpr int kMaxMacroLogL = 10;
+constexpr int kMaxOrder = 28;  // largest CFFT order
+
+// columns (butterflies) per workgroup of a radix-2^logl HBM pass
+constexpr int cfg_macro_c(int logl) { return logl <= 8 ? 32 : 16; }
+
+// radix (log2) of each HBM pass for a CFFT order above kMaxLdsOrder, first pass first
+struct MacroPlan {
+    int np;
+    int logl[4];
+};
+constexpr MacroPlan macro_plan(int order) {
+    switch (order) {
+        case 15: return {2, {8, 7}};
+        case 16: return {2, {8, 8}};
+        case 17: return {2, {9, 8}};
+        case 18: return {2, {9, 9}};
+        case 19: return {2, {10, 9}};
+        case 20: return {2, {10, 10}};
+        case 21: return {3, {7, 7, 7}};
+        case 22: return {3, {8, 7, 7}};
+        case 23: return {3, {8, 8, 7}};
+        case 24: return {3, {8, 8, 8}};
+        case 25: return {3, {9, 8, 8}};
+        case 26: return {3, {9, 9, 8}};
+        case 27: return {3, {9, 9, 9}};
+        case 28: return {3, {10, 9, 9}};
+        default: return {0, {0, 0, 0, 0}};
+    }
+}
+
+struct MacroEntry {
+    void (*fn)(MArgs);
+    int wg;        // threads per workgroup
+    int cols;      // butterflies (columns) per workgroup
+    int logl;      // log2 radix of the pass
+    int np;        // inner Stockham passes of the radix-L sub-FFT
+    int lr[8];
+    int lw[8];
+    int tw_total;  // inner twiddles
+};
+
+// [role][layout]: role 0 = first pass (layout = input), 1 = last pass
+// (layout = output), 2 = middle pass (layout unused, AoS)
+using MacroTable = MacroEntry[3][2];
+
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT>
+MacroEntry make_macro_entry() {
+    MacroEntry e{};
+    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT>;
+    e.wg = C * GL::T;
+    e.cols = C;
+    e.logl = GL::logn;
+    e.np = GL::NP;
+    for (int p = 0; p < GL::NP && p < 8; ++p) {
+        e.lr[p] = GL::lr(p);
+        e.lw[p] = GL::lw(p);
+    }
+    e.tw_total = GL::tw_total();
+    return e;
+}
+
 }  // namespace zl
+
+#define ZL_DECLARE_MACRO(n) void zl_fill_macro_##n(zl::MacroTable& tab);
+ZL_DECLARE_MACRO(6)
+ZL_DECLARE_MACRO(7)
+ZL_DECLARE_MACRO(8)
+ZL_DECLARE_MACRO(9)
+ZL_DECLARE_MACRO(10)
+#undef ZL_DECLARE_MACRO
 
 #define ZL_DECLARE_ORDER(n) void zl_fill_order_##n(zl::OrderTable& tab);
 ZL_DECLARE_ORDER(0)
