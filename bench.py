@@ -36,8 +36,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cfft4096_b16384.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
@@ -155,7 +155,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "cfft_f32_aos_n4096_batch16384_forward", "order": ORDER, "batch": BATCH,
-                   "layout": "aos->aos", "path": "plain-f32-lds-stockham"},
+                   "layout": "aos->aos", "path": "plain-f32-streaming-stockham"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes()},
         "rel_l2_max_vs_fp64": rel,

@@ -260,3 +260,38 @@ def test_batched_4096_point_config_properties():
     xs = x[idx].cpu().numpy().astype(np.complex128)
     ys = y[idx].cpu().numpy().astype(np.complex128)
     assert rel(ys, oracle_rows(xs, True, "f64")).max() <= TOL
+
+
+@pytest.mark.parametrize("order", [8, 10, 11, 12, 13, 14])
+def test_streaming_batches(order):
+    """Batches larger than the resident workgroups run the persistent
+    streaming kernel; every transform must match the oracle."""
+    n = 1 << order
+    batch = max(4096, (1 << 23) // n) + 3  # + 3: a partly filled last group
+    g = torch.Generator(device="cuda").manual_seed(order)
+    x = torch.randn((batch, n), dtype=torch.complex64, device="cuda", generator=g)
+    plan = zf.CFFT(order)
+    idx = np.unique(np.linspace(0, batch - 1, 12).astype(int))
+    xs = x[idx].cpu().numpy().astype(np.complex128)
+    for lin, lout in LAYOUTS:
+        xin = x if lin == zf.AOS else (x.real.contiguous(), x.imag.contiguous())
+        for forward in (True, False):
+            yo = make_out((batch, n), lout)
+            (plan.forward if forward else plan.backward)(xin, yo)
+            torch.cuda.synchronize()
+            ys = fetch(tuple(t[idx] for t in yo) if isinstance(yo, tuple) else yo[idx])
+            e = rel(ys, oracle_rows(xs, forward, "f64")).max()
+            print(f"stream order={order} batch={batch} lin={lin} lout={lout} fwd={forward} relL2={e:.3e}", flush=True)
+            assert e <= TOL
+    r = zf.RFFT(order + 1)
+    xr = torch.randn((batch, 2 * n), dtype=torch.float32, device="cuda", generator=g)
+    Y = torch.empty((batch, n + 1), dtype=torch.complex64, device="cuda")
+    S = torch.empty((batch, n + 1), dtype=torch.float32, device="cuda")
+    r.forward(xr, Y)
+    r.forward_sqr_mag(xr, S)
+    torch.cuda.synchronize()
+    xrs = xr[idx].cpu().numpy().astype(np.float64)
+    ref = np.stack([oracle.rfft_forward(v, "f64") for v in xrs])
+    assert rel(Y[idx].cpu().numpy(), ref).max() <= TOL
+    s64 = np.abs(ref) ** 2
+    assert (np.linalg.norm(S[idx].cpu().numpy() - s64, axis=1) / np.linalg.norm(s64, axis=1)).max() <= 2 * TOL

@@ -72,6 +72,27 @@ __global__ void gen_pass_twiddles(float2* out, int lr, int lw, int count) {
     out[idx] = make_float2((float)c, (float)s);
 }
 
+// LDS twiddle image of one Stockham pass (radix 2^lr, width 2^lw): the
+// direct table for w <= 16, else lo[(m-1)16 + b] = W_{rw}^{m b} followed by
+// hi[(m-1)(w/16) + a] = W_{rw}^{16 m a}; FP64 on the GPU, rounded to FP32.
+__global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi_n) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= lo_n + hi_n) return;
+    const long long w = 1ll << lw, rw = 1ll << (lr + lw);
+    long long e;
+    if (idx < lo_n) {
+        const long long s = w <= 16 ? w : 16;
+        e = (idx / s + 1) * (idx % s);
+    } else {
+        const long long j = idx - lo_n, s = w / 16;
+        e = 16 * (j / s + 1) * (j % s);
+    }
+    e &= rw - 1;
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)rw, &sn, &cs);
+    out[idx] = make_float2((float)cs, (float)sn);
+}
+
 // Macro twiddle table of an HBM pass: out[idx] = W_{2^log_lw}^{idx << shift},
 // FP64 on the GPU, rounded to FP32.
 __global__ void gen_macro_twiddles(float2* out, int log_lw, int shift, int count) {
@@ -105,6 +126,10 @@ struct zlfft_plan_s {
     int device = 0;
     const zl::OrderTable* tab = nullptr;  // LDS kernels (corder <= kMaxLdsOrder)
     float2* d_tw = nullptr;
+    float2* d_img = nullptr;    // LDS twiddle image for the streaming kernels
+    float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
+    int cus = 0;                // compute units of the device
+    int stream_blocks[4][2][2] = {};  // resident workgroups per CU of each streaming kernel
     float2* d_rtw = nullptr;
     int n_rtw = 0;
     // multi-pass transforms (corder > kMaxLdsOrder)
@@ -241,10 +266,19 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
     a.rtw = p->d_rtw;
     a.batch = (long long)batch;
     a.backward = backward;
-    const size_t grid = (batch + e.tpw - 1) / e.tpw;
-    if (grid > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
+    const size_t groups = (batch + e.tpw - 1) / e.tpw;
+    if (groups > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
     void* params[] = {&a};
-    ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)grid), dim3(e.wg), params, 0,
+    const int resident = e.stream_fn[0] ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
+    if (resident > 0 && groups > (size_t)resident) {
+        // persistent streaming kernel: one workgroup per resident slot
+        a.tw = p->d_img;
+        a.sink = p->d_sink;
+        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident), dim3(e.wg),
+                               params, 0, reinterpret_cast<hipStream_t>(stream)));
+        return ZLFFT_OK;
+    }
+    ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params, 0,
                            reinterpret_cast<hipStream_t>(stream)));
     return ZLFFT_OK;
 }
@@ -385,6 +419,37 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         e = hipGetLastError();
         if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
     }
+    if (!p->macro) {
+        e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
+        if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceGetAttribute"));
+        for (int k = 0; k < 4; ++k)
+            for (int i = 0; i < 2; ++i)
+                for (int o = 0; o < 2; ++o) {
+                    const zl::KernelEntry& ke = (*p->tab)[k][i][o];
+                    if (!ke.stream_fn[0]) continue;
+                    int nb = 0;
+                    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &nb, reinterpret_cast<const void*>(ke.stream_fn[0]), ke.wg, 0);
+                    if (e != hipSuccess) return cleanup(hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
+                    p->stream_blocks[k][i][o] = nb;
+                }
+        const zl::KernelEntry& ke = (*p->tab)[zl::C2C][0][0];
+        if (ke.stream_fn[0] || (*p->tab)[zl::R2C][0][0].stream_fn[0]) {
+            e = hipMalloc(&p->d_sink, (size_t)zl::kSinkBytes);
+            if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "sink hipMalloc failed"));
+        }
+        if (ke.img_total > 0) {
+            e = hipMalloc(&p->d_img, sizeof(float2) * (size_t)ke.img_total);
+            if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
+            size_t off = 0;
+            for (int q = 1; q < ke.np; ++q) {
+                const int cnt = ke.img_lo[q] + ke.img_hi[q];
+                gen_image_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_img + off, ke.lr[q], ke.lw[q],
+                                                                              ke.img_lo[q], ke.img_hi[q]);
+                off += (size_t)cnt;
+            }
+        }
+    }
     const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[zl::C2C][0][0];
     if (ent.tw_total > 0) {
         e = hipMalloc(&p->d_tw, sizeof(float2) * (size_t)ent.tw_total);
@@ -419,6 +484,8 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (!p) return fail(ZLFFT_ERR_NULL, "plan is NULL");
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->d_tw) (void)hipFree(p->d_tw);
+    if (p->d_img) (void)hipFree(p->d_img);
+    if (p->d_sink) (void)hipFree(p->d_sink);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
     if (p->d_stage) (void)hipFree(p->d_stage);
     if (p->d_work) (void)hipFree(p->d_work);

@@ -225,7 +225,12 @@ struct KArgs {
     const float2* rtw; // RFFT (wc, ws) for k = 1..M/2, at [k - 1]
     long long batch;
     int backward;      // C2C only
+    float2* sink;      // kSinkBytes of scratch that inactive lanes of the
+                       // streaming kernel read and write instead of user memory
 };
+
+// size of KArgs::sink: two arrays of the largest LDS transform (2^14 complex)
+constexpr long long kSinkBytes = 2ll * 8 * (1 << 14);
 
 __device__ __forceinline__ float2 ld_c(const void* p0, const void* p1, int lay, long long e) {
     if (lay == AOS) return reinterpret_cast<const float2*>(p0)[e];
@@ -396,6 +401,223 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
                 if (e < wn) st_c(a.out0, a.out1, lout, wb + e, lds[s * G::LDSN + G::pad(pos)]);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------- streaming kernel
+
+// Non-temporal global access: every input element is read once and every
+// output element written once per launch, so neither is kept in cache.
+typedef float zl_f2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float ld_nt(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+    const zl_f2v t = __builtin_nontemporal_load(reinterpret_cast<const zl_f2v*>(p));
+    return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void st_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+    const zl_f2v t = {v.x, v.y};
+    __builtin_nontemporal_store(t, reinterpret_cast<zl_f2v*>(p));
+}
+__device__ __forceinline__ float2 ld_c_nt(const void* p0, const void* p1, int lay, long long e) {
+    if (lay == AOS) return ld_nt(reinterpret_cast<const float2*>(p0) + e);
+    return make_float2(ld_nt(reinterpret_cast<const float*>(p0) + e), ld_nt(reinterpret_cast<const float*>(p1) + e));
+}
+__device__ __forceinline__ void st_c_nt(void* p0, void* p1, int lay, long long e, float2 v) {
+    if (lay == AOS) {
+        st_nt(reinterpret_cast<float2*>(p0) + e, v);
+    } else {
+        st_nt(reinterpret_cast<float*>(p0) + e, v.x);
+        st_nt(reinterpret_cast<float*>(p1) + e, v.y);
+    }
+}
+
+// LDS image of the Stockham twiddles of geometry G.  Pass p (radix r, width w)
+// needs W_{r w}^{m k}, m = 1..r-1, k < w.  For w <= 16 the image holds them
+// directly at [(m - 1) w + k]; for w > 16 it holds two factors,
+// lo[(m - 1) 16 + b] = W_{r w}^{m b} and hi[(m - 1) (w / 16) + a] = W_{r w}^{16 m a},
+// and W^{m k} = lo * hi with k = 16 a + b, so the image stays a few KiB.
+template <class G>
+struct TwImage {
+    static constexpr int lo_n(int p) {
+        return ((1 << G::lr(p)) - 1) * ((1 << G::lw(p)) <= 16 ? (1 << G::lw(p)) : 16);
+    }
+    static constexpr int hi_n(int p) { return (1 << G::lw(p)) <= 16 ? 0 : ((1 << G::lr(p)) - 1) * ((1 << G::lw(p)) / 16); }
+    static constexpr int off(int p) {
+        int o = 0;
+        for (int q = 1; q < p; ++q) o += lo_n(q) + hi_n(q);
+        return o;
+    }
+    static constexpr int total() { return G::NP > 1 ? off(G::NP) : 0; }
+};
+
+// Pass P with twiddles from the LDS image `ltw` (see TwImage).
+template <class G, int P>
+__device__ __forceinline__ void run_pass_img(float2 (&v)[G::R], float2* slds, const float2* ltw, int t) {
+    constexpr int lr = G::lr(P), r = 1 << lr, nb = G::R / r, lw = G::lw(P), w = 1 << lw;
+    constexpr bool last = (P == G::NP - 1);
+    using TI = TwImage<G>;
+#pragma unroll
+    for (int b = 0; b < nb; ++b) {
+        float2 x[r];
+#pragma unroll
+        for (int m = 0; m < r; ++m) x[m] = v[b + m * nb];
+        if constexpr (P > 0) {
+            const int i = t + G::T * b;
+            const int k = i & (w - 1);
+            if constexpr (w <= 16) {
+                const float2* twp = ltw + TI::off(P) + k;
+#pragma unroll
+                for (int m = 1; m < r; ++m) x[m] = cmul(x[m], twp[(m - 1) * w]);
+            } else {
+                const float2* lo = ltw + TI::off(P) + (k & 15);
+                const float2* hi = ltw + TI::off(P) + TI::lo_n(P) + (k >> 4);
+#pragma unroll
+                for (int m = 1; m < r; ++m) x[m] = cmul(x[m], cmul(lo[(m - 1) * 16], hi[(m - 1) * (w / 16)]));
+            }
+        }
+        dft<r>(x);
+#pragma unroll
+        for (int q = 0; q < r; ++q) v[b + q * nb] = x[q];
+    }
+    if constexpr (!last) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < nb; ++b) {
+            const int i = t + G::T * b;
+            const int base = ((i >> lw) << (lw + lr)) + (i & (w - 1));
+#pragma unroll
+            for (int q = 0; q < r; ++q) slds[G::pad(base + (q << lw))] = v[b + q * nb];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
+        run_pass_img<G, P + 1>(v, slds, ltw, t);
+    }
+}
+
+// Persistent streaming C2C / R2C kernel (direction BWD fixed at compile time)
+// for geometries whose direct I/O is
+// coalesced (T * element size >= 128 bytes or one transform per workgroup).
+// Each workgroup loops over groups of TPW transforms; while it computes one
+// group the next group's input is already in flight (register prefetch with
+// non-temporal loads), and twiddles come from an LDS image loaded once, so no
+// global load waits behind the prefetch.  Same arithmetic as fft_lds_kernel.
+#ifndef ZL_STREAM_MIN_WAVES
+#define ZL_STREAM_MIN_WAVES 1
+#endif
+template <class G, int KIND, int LIN, int LOUT, int BWD>
+__global__ void __launch_bounds__(G::WG) __attribute__((amdgpu_waves_per_eu(ZL_STREAM_MIN_WAVES, 8)))
+fft_stream_kernel(KArgs a) {
+    static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR, "streaming kernel handles C2C and R2C");
+    using TI = TwImage<G>;
+    __shared__ float2 lds[G::TPW * G::LDSN];
+    __shared__ float2 ltw[TI::total() > 0 ? TI::total() : 1];
+    __shared__ float2 lrtw[KIND == C2C ? 1 : (G::N / 2 > 0 ? G::N / 2 : 1)];
+    const int tid = threadIdx.x;
+    const int slot = tid / G::T;
+    const int t = tid - slot * G::T;
+    float2* slds = lds + slot * G::LDSN;
+    constexpr int M = G::N;
+    constexpr int lin = (KIND == C2C) ? LIN : AOS;
+    constexpr bool bwd = (KIND == C2C) && BWD != 0;  // direction fixed per instantiation
+    const long long ngroups = (a.batch + G::TPW - 1) / G::TPW;
+
+    for (int e = tid; e < TI::total(); e += G::WG) ltw[e] = a.tw[e];
+    if constexpr (KIND != C2C) {
+        for (int e = tid; e < M / 2; e += G::WG) lrtw[e] = a.rtw[e];
+    }
+
+    // Lanes without a transform (past the batch end, or past the last group)
+    // load from and store to the sink instead: every path of the loop then
+    // issues the same memory operations, with no branch around them, so the
+    // waits for the prefetch never wait for the stores issued after it.
+    const void* sk0 = a.sink;
+    const void* sk1 = reinterpret_cast<const float*>(a.sink) + (1 << 14);
+    float2 p[G::R];
+    long long g = blockIdx.x;
+    {
+        const long long f = g * G::TPW + slot;
+        const bool ok = g < ngroups && f < a.batch;
+        const void* b0 = ok ? a.in0 : sk0;
+        const void* b1 = ok ? a.in1 : sk1;
+        const long long eo = ok ? f * M : 0;
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt(b0, b1, lin, eo + t + G::T * j);
+    }
+    // One group: take the prefetched input, prefetch the next group, transform,
+    // store.  The first call is peeled off the loop so the loop is entered
+    // only from a copy of its own body: both ways into the loop header then
+    // have the prefetch loads followed by the stores in flight, and the
+    // header's wait covers the loads only.
+    auto group = [&](long long g) {
+        const long long f = g * G::TPW + slot;
+        const bool active = f < a.batch;
+        float2 v[G::R];
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) v[j] = bwd ? swapri(p[j]) : p[j];
+        {
+            const long long fn = (g + gridDim.x) * G::TPW + slot;
+            const bool ok = g + gridDim.x < ngroups && fn < a.batch;
+            const void* b0 = ok ? a.in0 : sk0;
+            const void* b1 = ok ? a.in1 : sk1;
+            const long long eo = ok ? fn * M : 0;
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt(b0, b1, lin, eo + t + G::T * j);
+        }
+        __syncthreads();  // LDS image ready (first group) / previous group's LDS readers done
+        if constexpr (G::NP > 0) run_pass_img<G, 0>(v, slds, ltw, t);
+        if constexpr (KIND == C2C) {
+            void* o0 = active ? a.out0 : a.sink;
+            void* o1 = active ? a.out1 : const_cast<void*>(sk1);
+            const long long eo = active ? f * M : 0;
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, LOUT, eo + t + G::T * j, bwd ? swapri(v[j]) : v[j]);
+        } else {
+            // post-twiddle epilogue, identical to fft_lds_kernel's
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
+            __syncthreads();
+            if (active) {
+                const long long fb = f * (M + 1);
+                if (t == 0) {
+                    const float2 z0 = slds[0];
+                    const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+                    if constexpr (KIND == R2C) {
+                        st_c_nt(a.out0, a.out1, LOUT, fb, make_float2(dc, 0.f));
+                        st_c_nt(a.out0, a.out1, LOUT, fb + M, make_float2(ny, 0.f));
+                    } else {
+                        float* o = reinterpret_cast<float*>(a.out0);
+                        st_nt(o + fb, dc * dc);
+                        st_nt(o + fb + M, ny * ny);
+                    }
+                }
+                for (int k = 1 + t; k <= M / 2; k += G::T) {
+                    const float2 z1 = slds[G::pad(k)];
+                    const float2 z2 = slds[G::pad(M - k)];
+                    const float2 w = lrtw[k - 1];
+                    const float si = z1.y + z2.y, dr = z1.x - z2.x;
+                    const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
+                    const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
+                    const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
+                    const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
+                    if constexpr (KIND == R2C) {
+                        if (k != M - k) st_c_nt(a.out0, a.out1, LOUT, fb + k, make_float2(xr_k, xi_k));
+                        st_c_nt(a.out0, a.out1, LOUT, fb + (M - k), make_float2(xr_m, xi_m));
+                    } else {
+                        float* o = reinterpret_cast<float*>(a.out0);
+                        if (k != M - k) st_nt(o + fb + k, __builtin_fmaf(xr_k, xr_k, xi_k * xi_k));
+                        st_nt(o + fb + (M - k), __builtin_fmaf(xr_m, xr_m, xi_m * xi_m));
+                    }
+                }
+            }
+        }
+    };
+    if (g < ngroups) {
+        group(g);
+        for (g += gridDim.x; g < ngroups; g += gridDim.x) group(g);
     }
 }
 

@@ -14,13 +14,17 @@ constexpr int cfg_tpb(int) { return 256; }
 
 struct KernelEntry {
     void (*fn)(KArgs);
+    void (*stream_fn[2])(KArgs);  // persistent streaming variant [forward, backward], or null
     int wg;        // threads per workgroup
     int tpw;       // transforms per workgroup
     int logn;      // CFFT order
     int np;        // Stockham passes
     int lr[8];     // log2 radix per pass
     int lw[8];     // log2 width per pass
-    int tw_total;  // complex twiddles of all passes
+    int tw_total;  // complex twiddles of all passes (direct table)
+    int img_total; // complex twiddles of the LDS image (streaming variant)
+    int img_lo[8]; // per pass: entries of the direct / low factor part
+    int img_hi[8]; // per pass: entries of the high factor part (0 if direct)
 };
 
 // table[kind][lin][lout] for one order; unused combinations stay null
@@ -39,6 +43,19 @@ KernelEntry make_entry() {
         e.lw[p] = G::lw(p);
     }
     e.tw_total = G::tw_total();
+    constexpr int elem = (KIND == C2C && (LIN == SOA || LOUT == SOA)) ? 4 : 8;
+    constexpr bool direct = G::T * elem >= 128 || G::TPW == 1;
+    constexpr long long stream_lds =
+        8ll * (G::TPW * G::LDSN + TwImage<G>::total() + (KIND == C2C ? 1 : G::N / 2));
+    if constexpr (KIND != C2R && direct && G::NP >= 2 && stream_lds <= 160 * 1024) {
+        e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0>;
+        if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1>;
+        e.img_total = TwImage<G>::total();
+        for (int p = 1; p < G::NP && p < 8; ++p) {
+            e.img_lo[p] = TwImage<G>::lo_n(p);
+            e.img_hi[p] = TwImage<G>::hi_n(p);
+        }
+    }
     return e;
 }
 
