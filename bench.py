@@ -4,7 +4,8 @@ Workload (one "step"): one forward pass of 16384 complex FP32 CFFTs of
 N = 4096 points, AoS in and AoS out (1 GiB of algorithmic HBM traffic:
 16 B per point, one read of the input and one write of the output), with the
 inputs resident in HBM.  The step is one launch of the LDS-resident kernel
-through the C-ABI (zlfft_cfft_exec) on torch's current stream.
+through the C-ABI (zlfft_cfft_exec) on torch's current stream: the persistent
+streaming Stockham kernel for N = 4096.
 
 Printed: ONE JSON line with whole-job throughput in GB/s of algorithmic
 traffic, the kernel's roofline fraction against the 8 TB/s HBM peak (average
@@ -30,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 ORDER, BATCH = 12, 16384
 N = 1 << ORDER
 BYTES_PER_STEP = 16 * N * BATCH  # read N complex64 + write N complex64, per transform
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cfft4096_b16384.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cfft4096_b16384_pmc.md")
 
 
 def parse():
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
     return ap.parse_args()
 
 
@@ -77,10 +80,12 @@ def traffic_bytes():
     WRITE_SIZE, scaled by a copy kernel of known size), if profiled."""
     try:
         with open(TRAFFIC_FILE) as f:
-            d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
+            for line in f:
+                if line.startswith("hbm_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
     except (OSError, ValueError):
-        return None
+        pass
+    return None
 
 
 def main():
@@ -111,6 +116,11 @@ def main():
         if rc:
             raise RuntimeError(lib.zlfft_last_error().decode())
 
+    if args.copy_probe:
+        # known-size copy: reads and writes BATCH * N * 8 bytes, for scaling HBM counters
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
