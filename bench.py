@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA split-precision variants")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
     return ap.parse_args()
@@ -73,6 +74,48 @@ def cpu_baseline(seconds: float) -> dict:
                   f"oracle FP32 restatement of the reference ({path} plan, switch order {sw})",
         "transforms_per_s": round(sample / t, 1),
     }
+
+
+MFMA_VARIANTS = (("f16_split", 2), ("bf16_split3", 5), ("f16", 1), ("bf16", 3), ("bf16_split", 4))
+
+
+def mfma_variants(x, y, xs, ref, stream, lib, args) -> dict:
+    """The same workload on the MFMA split-precision kernels: each radix-16
+    pass as a 16x16 DFT-matrix product on FP16 / BF16 matrix cores with FP32
+    accumulate.  Throughput (event-timed like the main line) and accuracy vs
+    the FP64 oracle on the same sampled transforms."""
+    import torch
+    import zldsp_fft_amd as zf
+    res = {}
+    idx = list(range(0, BATCH, BATCH // 16))
+    steps = args.steps
+    for name, prec in MFMA_VARIANTS:
+        plan = zf.CFFT(ORDER, prec)
+        h, sp = plan._h, stream.cuda_stream
+
+        def step():
+            rc = lib.zlfft_cfft_exec(h, zf.FORWARD, zf.AOS, x.data_ptr(), None, zf.AOS, y.data_ptr(), None, BATCH, sp)
+            if rc:
+                raise RuntimeError(lib.zlfft_last_error().decode())
+
+        for _ in range(max(20, args.warmup)):
+            step()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(steps):
+            step()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        ys = y[idx].cpu().numpy().astype(np.complex128)
+        rel = float((np.linalg.norm(ys - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
+        mbe = float((np.abs(ys - ref).max(axis=1) / np.abs(ref).max(axis=1)).max())
+        gbs = BYTES_PER_STEP / (ms * 1e-3) / 1e9
+        res[name] = {"GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "rel_l2_max": rel,
+                     "max_bin_err": mbe, "meets_1e-6": rel <= 1e-6}
+        plan.close()
+    return res
 
 
 def traffic_bytes():
@@ -173,6 +216,8 @@ def main():
         "gflops_5nlog2n": round(5 * N * ORDER * BATCH / (ms_per_step * 1e-3) / 1e9, 1),
         "kernel_ms_avg": round(kern_ms, 5),
     }
+    if not args.no_mfma:
+        out["mfma"] = mfma_variants(x, y, xs, ref, stream, lib, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     if rel > 1e-6:

@@ -60,8 +60,18 @@ extern "C" {
 #define ZLFFT_KIND_CFFT 0
 #define ZLFFT_KIND_RFFT 1
 
-/* arithmetic of a plan */
+/* arithmetic of a plan.  ZLFFT_PREC_F32 is the plain path, for every kind,
+ * order and layout.  The MFMA precisions run each radix-16 pass of the
+ * 4096-point CFFT (order 12, AoS in and out) as a 16x16 DFT-matrix product on
+ * the matrix cores with FP32 accumulate; the operands are FP16 or BF16, whole
+ * or split into 2 (hi + lo) or 3 (hi + mid + lo) parts.  Other orders, kinds or
+ * layouts with an MFMA precision return ZLFFT_ERR_UNSUPPORTED. */
 #define ZLFFT_PREC_F32 0
+#define ZLFFT_PREC_F16_MFMA 1
+#define ZLFFT_PREC_F16_MFMA_SPLIT 2
+#define ZLFFT_PREC_BF16_MFMA 3
+#define ZLFFT_PREC_BF16_MFMA_SPLIT 4
+#define ZLFFT_PREC_BF16_MFMA_SPLIT3 5
 
 /* user layouts */
 #define ZLFFT_AOS 0

@@ -37,6 +37,8 @@ lib_path = os.path.join(_HERE, "libzlfft.so")
 OK = 0
 KIND_CFFT, KIND_RFFT = 0, 1
 PREC_F32 = 0
+PREC_F16_MFMA, PREC_F16_MFMA_SPLIT = 1, 2
+PREC_BF16_MFMA, PREC_BF16_MFMA_SPLIT, PREC_BF16_MFMA_SPLIT3 = 3, 4, 5
 AOS, SOA = 0, 1
 FORWARD, BACKWARD = 0, 1
 MAX_CFFT_ORDER = 28
@@ -193,12 +195,12 @@ def _batch(n_have: int, n_per: int, name: str) -> int:
 # ---------------------------------------------------------------- plans
 
 class _Plan:
-    def __init__(self, kind: int, order: int):
+    def __init__(self, kind: int, order: int, precision: int = PREC_F32):
         if isinstance(order, bool) or not isinstance(order, (int, np.integer)) or order < 0:
             raise ValueError("order must be a non-negative integer")
         self._lib = load_library()
         h = ctypes.c_void_p()
-        _check(self._lib.zlfft_plan_create(ctypes.byref(h), kind, PREC_F32, int(order)))
+        _check(self._lib.zlfft_plan_create(ctypes.byref(h), kind, int(precision), int(order)))
         self._h = h
 
     def close(self) -> None:
@@ -230,10 +232,13 @@ class _Plan:
 
 
 class CFFT(_Plan):
-    """``zldsp::fft::CFFT<float>`` (reference ``src/zldsp_fft_cfft.hpp:11-118``)."""
+    """``zldsp::fft::CFFT<float>`` (reference ``src/zldsp_fft_cfft.hpp:11-118``).
 
-    def __init__(self, order: int):
-        super().__init__(KIND_CFFT, order)
+    ``precision`` selects the arithmetic: PREC_F32 (the plain path) or one of the
+    MFMA split precisions (4096-point AoS transforms only)."""
+
+    def __init__(self, order: int, precision: int = PREC_F32):
+        super().__init__(KIND_CFFT, order, precision)
 
     def _run(self, direction: int, in_buffer, out_buffer) -> None:
         n = self.get_size()

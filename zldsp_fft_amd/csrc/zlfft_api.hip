@@ -93,6 +93,16 @@ __global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi
     out[idx] = make_float2((float)cs, (float)sn);
 }
 
+// FP16 MFMA kernels' twiddle image: every twiddle W of the image divided by
+// 16 (exact), i.e. the direct / low-factor entries [0, lo_n) of one pass
+// scaled and the high-factor entries [lo_n, lo_n + hi_n) copied.
+__global__ void scale_twiddles16(const float2* in, float2* out, int lo_n, int hi_n) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= lo_n + hi_n) return;
+    const float s = idx < lo_n ? 0.0625f : 1.0f;
+    out[idx] = make_float2(in[idx].x * s, in[idx].y * s);
+}
+
 // Macro twiddle table of an HBM pass: out[idx] = W_{2^log_lw}^{idx << shift},
 // FP64 on the GPU, rounded to FP32.
 __global__ void gen_macro_twiddles(float2* out, int log_lw, int shift, int count) {
@@ -128,6 +138,7 @@ struct zlfft_plan_s {
     float2* d_tw = nullptr;
     float2* d_img = nullptr;    // LDS twiddle image for the streaming kernels
     float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
+    float2* d_img16 = nullptr;  // twiddle image / 16 for the FP16 MFMA kernels
     int cus = 0;                // compute units of the device
     int stream_blocks[4][2][2] = {};  // resident workgroups per CU of each streaming kernel
     float2* d_rtw = nullptr;
@@ -251,9 +262,29 @@ int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, con
     return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work);
 }
 
+int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, size_t batch, int backward,
+                void* stream) {
+    if (lin != ZLFFT_AOS || lout != ZLFFT_AOS)
+        return fail(ZLFFT_ERR_UNSUPPORTED, "the MFMA precisions take AoS input and output");
+    void* fn = zl_mfma_kernel(p->precision, backward);
+    if (!fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no MFMA kernel for this precision");
+    if (batch == 0) return ZLFFT_OK;
+    if (batch > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
+    zl::KArgs a{};
+    a.in0 = in0;
+    a.out0 = out0;
+    const bool f16 = p->precision == ZLFFT_PREC_F16_MFMA || p->precision == ZLFFT_PREC_F16_MFMA_SPLIT;
+    a.tw = f16 ? p->d_img16 : p->d_img;
+    a.batch = (long long)batch;
+    void* params[] = {&a};
+    ZL_HIP(hipLaunchKernel(fn, dim3((unsigned)batch), dim3(256), params, 0, reinterpret_cast<hipStream_t>(stream)));
+    return ZLFFT_OK;
+}
+
 int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
            size_t batch, int backward, void* stream) {
     if (p->macro) return launch_macro(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream);
+    if (p->precision != ZLFFT_PREC_F32) return launch_mfma(p, lin, lout, in0, out0, batch, backward, stream);
     const zl::KernelEntry& e = (*p->tab)[kind][lin][lout];
     if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no kernel for this kind/layout combination");
     if (batch == 0) return ZLFFT_OK;
@@ -359,7 +390,10 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     if (!plan) return fail(ZLFFT_ERR_NULL, "plan out-pointer is NULL");
     *plan = nullptr;
     if (kind != ZLFFT_KIND_CFFT && kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "unknown plan kind");
-    if (precision != ZLFFT_PREC_F32) return fail(ZLFFT_ERR_UNSUPPORTED, "only ZLFFT_PREC_F32 is supported");
+    if (precision < ZLFFT_PREC_F32 || precision > ZLFFT_PREC_BF16_MFMA_SPLIT3)
+        return fail(ZLFFT_ERR_UNSUPPORTED, "unknown precision");
+    if (precision != ZLFFT_PREC_F32 && (kind != ZLFFT_KIND_CFFT || order != 12))
+        return fail(ZLFFT_ERR_UNSUPPORTED, "the MFMA precisions cover the 4096-point CFFT (order 12) only");
     if (kind == ZLFFT_KIND_RFFT && order < 1)
         return fail(ZLFFT_ERR_INVALID_ORDER, "RFFT order must be >= 1 (the reference underflows at 0)");
     const size_t corder = kind == ZLFFT_KIND_RFFT ? order - 1 : order;
@@ -448,6 +482,17 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                                                                               ke.img_lo[q], ke.img_hi[q]);
                 off += (size_t)cnt;
             }
+            if (precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT) {
+                e = hipMalloc(&p->d_img16, sizeof(float2) * (size_t)ke.img_total);
+                if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
+                size_t o16 = 0;
+                for (int q = 1; q < ke.np; ++q) {
+                    const int cnt = ke.img_lo[q] + ke.img_hi[q];
+                    scale_twiddles16<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_img + o16, p->d_img16 + o16,
+                                                                               ke.img_lo[q], ke.img_hi[q]);
+                    o16 += (size_t)cnt;
+                }
+            }
         }
     }
     const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[zl::C2C][0][0];
@@ -486,6 +531,7 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (p->d_tw) (void)hipFree(p->d_tw);
     if (p->d_img) (void)hipFree(p->d_img);
     if (p->d_sink) (void)hipFree(p->d_sink);
+    if (p->d_img16) (void)hipFree(p->d_img16);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
     if (p->d_stage) (void)hipFree(p->d_stage);
     if (p->d_work) (void)hipFree(p->d_work);

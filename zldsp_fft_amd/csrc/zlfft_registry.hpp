@@ -126,6 +126,9 @@ MacroEntry make_macro_entry() {
 
 }  // namespace zl
 
+// MFMA split-precision 4096-point kernel for a precision code, or null
+void* zl_mfma_kernel(int prec, int backward);
+
 #define ZL_DECLARE_MACRO(n) void zl_fill_macro_##n(zl::MacroTable& tab);
 ZL_DECLARE_MACRO(6)
 ZL_DECLARE_MACRO(7)
