@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA split-precision variants")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
     return ap.parse_args()
@@ -116,6 +117,25 @@ def mfma_variants(x, y, xs, ref, stream, lib, args) -> dict:
                      "max_bin_err": mbe, "meets_1e-6": rel <= 1e-6}
         plan.close()
     return res
+
+
+def ragged_section(args) -> dict:
+    """The ragged batch: 10,000 transforms with orders uniform in 6..14,
+    CFFT / RFFT, forward / backward, AoS / SoA (50/50 each), in one
+    graph-launched persistent kernel.  Throughput in algorithmic bytes (one
+    read of each input, one write of each output) per graph launch."""
+    import torch
+    import zldsp_fft_amd as zf
+    from zldsp_fft_amd import workloads
+    descs = workloads.ragged_mix(10000, 6)
+    nbytes = sum(workloads.algorithmic_bytes(d) for d in descs)
+    rb = zf.RaggedBatch(descs)
+    ms = workloads.time_ms(torch, rb.run, reps=max(20, args.steps // 4), warmup=10)
+    rb.close()
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"workload": "ragged_10000_orders6-14_mixed", "bytes_per_launch": nbytes, "ms_per_launch": round(ms, 5),
+            "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "transforms_per_s": round(len(descs) / (ms * 1e-3), 1)}
 
 
 def traffic_bytes():
@@ -218,6 +238,10 @@ def main():
     }
     if not args.no_mfma:
         out["mfma"] = mfma_variants(x, y, xs, ref, stream, lib, args)
+    if not args.no_ragged:
+        del x, y
+        torch.cuda.empty_cache()
+        out["ragged"] = ragged_section(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     if rel > 1e-6:

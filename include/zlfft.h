@@ -128,6 +128,39 @@ int zlfft_rfft_forward(zlfft_plan plan, const float *in, int out_layout, void *o
 int zlfft_rfft_forward_sqr_mag(zlfft_plan plan, const float *in, float *out);
 int zlfft_rfft_backward(zlfft_plan plan, int in_layout, const void *in0, const void *in1, float *out);
 
+/* ---- ragged batches ----
+ * Many transforms of different orders, kinds, directions and layouts run as
+ * one persistent kernel, captured once in a hipGraph: zlfft_ragged_exec is a
+ * single graph launch.  Each descriptor is one call of the reference API
+ * (CFFT forward/backward, RFFT forward/backward/forward_sqr_mag) on its own
+ * device buffers; the set is fixed at creation (the graph holds it).
+ * CFFT orders 3..14 and RFFT orders 4..15. */
+#define ZLFFT_FORWARD_SQR_MAG 2
+
+typedef struct {
+    int kind;      /* ZLFFT_KIND_CFFT or ZLFFT_KIND_RFFT */
+    int order;     /* size 2^order (complex for CFFT, real for RFFT) */
+    int direction; /* ZLFFT_FORWARD, ZLFFT_BACKWARD; RFFT also ZLFFT_FORWARD_SQR_MAG */
+    int layout;    /* ZLFFT_AOS or ZLFFT_SOA: both sides of a CFFT, the complex side of an RFFT */
+    const void *in0, *in1; /* device input (in1: SoA imaginary part) */
+    void *out0, *out1;     /* device output (out1: SoA imaginary part) */
+} zlfft_ragged_desc;
+
+typedef struct zlfft_ragged_s *zlfft_ragged;
+
+/* Builds the work queue, twiddles and the captured graph (no counterpart in
+ * the reference, whose calls are one transform each). */
+int zlfft_ragged_create(zlfft_ragged *ragged, const zlfft_ragged_desc *descs, size_t count);
+/* As zlfft_ragged_create, with the points per thread (log2: 3 or 4) of each
+ * CFFT order 0..14 given in logr[15] instead of the tuned table
+ * (zlfft_tune.txt next to the library, or $ZLFFT_TUNE_FILE); for the
+ * autotuner. */
+int zlfft_ragged_create_config(zlfft_ragged *ragged, const zlfft_ragged_desc *descs, size_t count,
+                               const int *logr);
+/* Runs every transform of the set: one hipGraphLaunch on `stream`. */
+int zlfft_ragged_exec(zlfft_ragged ragged, void *stream);
+int zlfft_ragged_destroy(zlfft_ragged ragged);
+
 /* Copies the plan's RFFT post/pre twiddles (wc, ws) for k = 1..N/4 as 2*(N/4)
  * floats to host memory (reference src/common/zldsp_fft_common_init.hpp:347-388);
  * for tests that compare them with the FP64 oracle. */

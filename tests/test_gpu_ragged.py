@@ -1,0 +1,149 @@
+"""Ragged batches: 10,000 transforms with orders uniform in 6..14, half CFFT
+and half RFFT, half forward and half backward, half AoS and half SoA, inputs
+uniform in [-1, 1), in one persistent graph-captured launch.  Every output
+must match the per-transform plain path within rel-L2 1e-6 (and a sample the
+FP64 oracle); replays of the graph give identical results."""
+import numpy as np
+import pytest
+
+import oracle
+import zldsp_fft_amd as zf
+from zldsp_fft_amd import signals, workloads
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+TOL = 1e-6
+
+
+def as_np(b):
+    if isinstance(b, tuple):
+        return b[0].cpu().numpy().astype(np.float64) + 1j * b[1].cpu().numpy().astype(np.float64)
+    return b.cpu().numpy()
+
+
+def plain(desc, plans):
+    kind, order, direction, layout, inb, outb = desc
+    key = (kind, order)
+    if key not in plans:
+        plans[key] = zf.CFFT(order) if kind == zf.KIND_CFFT else zf.RFFT(order)
+    p = plans[key]
+    out = (tuple(torch.empty_like(t) for t in outb) if isinstance(outb, tuple) else torch.empty_like(outb))
+    if kind == zf.KIND_CFFT:
+        (p.forward if direction == zf.FORWARD else p.backward)(inb, out)
+    else:
+        (p.forward if direction == zf.FORWARD else p.backward)(inb, out)
+    return out
+
+
+def test_ragged_mixed_10000():
+    descs = workloads.ragged_mix(10000, 6)
+    rb = zf.RaggedBatch(descs)
+    rb.run()
+    torch.cuda.synchronize()
+    first = [as_np(d[5]) for d in descs]
+    # replays: identical results (graph + queue counter reset by a kernel node)
+    for _ in range(2):
+        for d in descs:
+            o = d[5]
+            for t in (o if isinstance(o, tuple) else (o,)):
+                t.fill_(float("nan"))
+        rb.run()
+        torch.cuda.synchronize()
+        for d, f in zip(descs, first):
+            assert np.array_equal(as_np(d[5]), f, equal_nan=False)
+    plans = {}
+    worst = 0.0
+    for i, d in enumerate(descs):
+        ref = as_np(plain(d, plans))
+        got = first[i]
+        e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        worst = max(worst, e)
+        assert e <= TOL, (i, d[:4], e)
+    print(f"ragged 10000 vs plain path: max relL2 {worst:.3e}", flush=True)
+    # a sample against the FP64 oracle
+    for i in range(0, 10000, 997):
+        kind, order, direction, layout, inb, outb = descs[i]
+        x = as_np(inb)
+        n = 1 << order
+        if kind == zf.KIND_CFFT:
+            ref = oracle.cfft(x.astype(np.complex128), direction == zf.FORWARD, "f64")
+        elif direction == zf.FORWARD:
+            ref = oracle.rfft_forward(x.astype(np.float64), "f64")
+        else:
+            ref = oracle.rfft_backward(x.astype(np.complex128), n, "f64")
+        e = float(np.linalg.norm(first[i] - ref) / np.linalg.norm(ref))
+        assert e <= TOL, (i, descs[i][:4], e)
+
+
+def test_ragged_sqr_mag_and_small_orders():
+    descs = []
+    g = torch.Generator(device="cuda").manual_seed(1)
+    for order in range(4, 16):
+        n = 1 << order
+        x = torch.rand(n, generator=g, device="cuda") * 2 - 1
+        descs.append((zf.KIND_RFFT, order, zf.FORWARD_SQR_MAG, zf.AOS, x, torch.empty(n // 2 + 1, device="cuda")))
+    for order in range(3, 15):
+        n = 1 << order
+        x = torch.view_as_complex((torch.rand((n, 2), generator=g, device="cuda") * 2 - 1).contiguous())
+        descs.append((zf.KIND_CFFT, order, zf.BACKWARD, zf.AOS, x, torch.empty(n, dtype=torch.complex64, device="cuda")))
+    rb = zf.RaggedBatch(descs)
+    rb.run()
+    torch.cuda.synchronize()
+    for kind, order, direction, layout, inb, outb in descs:
+        n = 1 << order
+        x = inb.cpu().numpy()
+        if kind == zf.KIND_RFFT:
+            ref = np.abs(oracle.rfft_forward(x.astype(np.float64), "f64")) ** 2
+        else:
+            ref = oracle.cfft(x.astype(np.complex128), False, "f64")
+        got = outb.cpu().numpy()
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 2 * TOL, (kind, order)
+
+
+def test_ragged_rejects_bad_descriptors():
+    x = torch.zeros(64, dtype=torch.complex64, device="cuda")
+    with pytest.raises(zf.ZlfftError) as e:
+        zf.RaggedBatch([(zf.KIND_CFFT, 2, zf.FORWARD, zf.AOS, x, x)])
+    assert e.value.code == 1
+    with pytest.raises(zf.ZlfftError) as e:
+        zf.RaggedBatch([(zf.KIND_CFFT, 15, zf.FORWARD, zf.AOS, x, x)])
+    assert e.value.code == 1
+    with pytest.raises(zf.ZlfftError) as e:  # 8 points per thread: 2^14 / 8 threads exceed one workgroup
+        zf.RaggedBatch([(zf.KIND_CFFT, 14, zf.FORWARD, zf.AOS, x, x)], logr=[3] * 15)
+    assert e.value.code == 5
+    with pytest.raises(zf.ZlfftError) as e:
+        zf.RaggedBatch([(zf.KIND_CFFT, 6, zf.FORWARD_SQR_MAG, zf.AOS, x, x)])
+    assert e.value.code == 5
+
+
+def test_ragged_points_per_thread_choices_agree():
+    """Both points-per-thread configurations the autotuner chooses between
+    give the same results to 1e-6 on every order they support."""
+    for order in (3, 6, 9, 12, 13):
+        descs = workloads.ragged_uniform_order(order, 9, seed=order)
+        res = {}
+        for logr in (3, 4):
+            if logr == 4 and order < 4:
+                continue
+            table = [3 if n <= 3 else 4 for n in range(15)]
+            table[order] = logr  # also the CFFT order of the RFFT entries (real size 2^(order + 1))
+            rb = zf.RaggedBatch(descs, logr=table)
+            rb.run()
+            torch.cuda.synchronize()
+            res[logr] = [as_np(d[5]) for d in descs]
+            rb.close()
+        if len(res) == 2:
+            for a, b in zip(res[3], res[4]):
+                assert np.linalg.norm(a - b) / np.linalg.norm(b) <= TOL, order
+        for d, got in zip(descs, res[min(res)]):
+            kind, o, direction, layout, inb, outb = d
+            x = as_np(inb)
+            if kind == zf.KIND_CFFT:
+                ref = oracle.cfft(x.astype(np.complex128), direction == zf.FORWARD, "f64")
+            elif direction == zf.FORWARD:
+                ref = oracle.rfft_forward(x.astype(np.float64), "f64")
+            elif direction == zf.FORWARD_SQR_MAG:
+                ref = np.abs(oracle.rfft_forward(x.astype(np.float64), "f64")) ** 2
+            else:
+                ref = oracle.rfft_backward(x.astype(np.complex128), 1 << o, "f64")
+            assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 2 * TOL, (order, d[:4])

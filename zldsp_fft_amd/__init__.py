@@ -51,7 +51,16 @@ EXPORTS = (
     "zlfft_cfft_exec", "zlfft_rfft_forward_exec", "zlfft_rfft_forward_sqr_mag_exec", "zlfft_rfft_backward_exec",
     "zlfft_cfft_execute", "zlfft_rfft_forward", "zlfft_rfft_forward_sqr_mag", "zlfft_rfft_backward",
     "zlfft_plan_copy_rfft_twiddles", "zlfft_last_error", "zlfft_version",
+    "zlfft_ragged_create", "zlfft_ragged_create_config", "zlfft_ragged_exec", "zlfft_ragged_destroy",
 )
+FORWARD_SQR_MAG = 2
+
+
+class RaggedDesc(ctypes.Structure):
+    """zlfft_ragged_desc of include/zlfft.h"""
+    _fields_ = [("kind", ctypes.c_int), ("order", ctypes.c_int), ("direction", ctypes.c_int),
+                ("layout", ctypes.c_int), ("in0", ctypes.c_void_p), ("in1", ctypes.c_void_p),
+                ("out0", ctypes.c_void_p), ("out1", ctypes.c_void_p)]
 
 
 class ZlfftError(RuntimeError):
@@ -88,6 +97,11 @@ def load_library() -> ctypes.CDLL:
     L.zlfft_rfft_backward.argtypes = [vp, ci, vp, vp, vp]
     L.zlfft_plan_copy_rfft_twiddles.argtypes = [vp, vp, sz]
     L.zlfft_last_error.restype = ctypes.c_char_p
+    L.zlfft_ragged_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(RaggedDesc), sz]
+    L.zlfft_ragged_create_config.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(RaggedDesc), sz,
+                                             ctypes.POINTER(ctypes.c_int)]
+    L.zlfft_ragged_exec.argtypes = [vp, vp]
+    L.zlfft_ragged_destroy.argtypes = [vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name not in ("zlfft_plan_get_size", "zlfft_plan_get_order", "zlfft_last_error"):
@@ -325,3 +339,55 @@ class RFFT(_Plan):
         out = np.zeros(2 * m, dtype=np.float32)
         _check(self._lib.zlfft_plan_copy_rfft_twiddles(self._h, out.ctypes.data, out.size))
         return out.reshape(m, 2)
+
+
+class RaggedBatch:
+    """Many transforms of different orders, kinds, directions and layouts as
+    one persistent kernel in a captured hipGraph (``zlfft_ragged_*``).
+
+    ``descs`` is a sequence of ``(kind, order, direction, layout, in, out)``
+    where ``in`` / ``out`` are device tensors (AoS: one tensor; SoA: a
+    ``(real, imag)`` pair) exactly as the corresponding ``CFFT`` / ``RFFT``
+    call takes them.  The tensors must stay alive as long as the batch.
+    """
+
+    def __init__(self, descs, logr=None):
+        """``logr``: optional points per thread (log2, 3 or 4) for CFFT orders
+        0..14 instead of the tuned table."""
+        self._lib = load_library()
+        arr = (RaggedDesc * max(1, len(descs)))()
+        self._keep = []
+        for i, (kind, order, direction, layout, inb, outb) in enumerate(descs):
+            def ptrs(b):
+                if isinstance(b, (tuple, list)):
+                    return _ptr(b[0]), _ptr(b[1])
+                return _ptr(b), None
+            i0, i1 = ptrs(inb)
+            o0, o1 = ptrs(outb)
+            arr[i] = RaggedDesc(kind, order, direction, layout, i0, i1, o0, o1)
+            self._keep.append((inb, outb))
+        h = ctypes.c_void_p()
+        if logr is None:
+            _check(self._lib.zlfft_ragged_create(ctypes.byref(h), arr, len(descs)))
+        else:
+            lr = (ctypes.c_int * 15)(*logr)
+            _check(self._lib.zlfft_ragged_create_config(ctypes.byref(h), arr, len(descs), lr))
+        self._h = h
+
+    def run(self, stream: int | None = None) -> None:
+        """Runs every transform: one graph launch on `stream` (default: torch's current stream)."""
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(self._lib.zlfft_ragged_exec(self._h, stream))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            self._lib.zlfft_ragged_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

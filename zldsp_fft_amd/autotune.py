@@ -1,0 +1,102 @@
+"""Autotuner: per CFFT order, the ragged kernel's points per thread (8 or 16,
+i.e. the radix of its passes, the threads per transform and the LDS tile per
+workgroup), measured on this GPU and written to zlfft_tune.txt next to
+libzlfft.so, where plans read it.
+
+    python -m zldsp_fft_amd.autotune [--out PATH] [--points 2**24]
+
+For every order, a batch of one size that cycles through every variant
+(CFFT / RFFT x forward / backward x AoS / SoA, sqr-mag) is timed with each
+legal choice; the two choices' outputs must agree (rel-L2 <= 1e-6) before one
+is kept.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+
+import zldsp_fft_amd as zf  # noqa: E402
+from zldsp_fft_amd import workloads  # noqa: E402
+
+
+def _legal(order: int, logr: int) -> bool:
+    return (logr == 4 and order >= 4) or (logr == 3 and order >= 3 and (1 << (order - 3)) <= 1024)
+
+
+def _outputs(torch, descs):
+    res = []
+    for d in descs:
+        o = d[5]
+        res.append(tuple(t.clone() for t in o) if isinstance(o, tuple) else o.clone())
+    return res
+
+
+def _rel(a, b):
+    import torch
+    if isinstance(a, tuple):
+        num = sum(torch.linalg.vector_norm((x - y).double()) ** 2 for x, y in zip(a, b)) ** 0.5
+        den = sum(torch.linalg.vector_norm(y.double()) ** 2 for y in b) ** 0.5
+    else:
+        num = torch.linalg.vector_norm((a - b).to(torch.complex128) if a.is_complex() else (a - b).double())
+        den = torch.linalg.vector_norm(b.to(torch.complex128) if b.is_complex() else b.double())
+    return float(num / den)
+
+
+def tune(points: int, out_path: str, verbose: bool = True) -> dict:
+    import torch
+    results = {}
+    base = [3 if n <= 3 else 4 for n in range(15)]
+    for order in range(3, 15):
+        count = max(9, points >> order)
+        descs = workloads.ragged_uniform_order(order, count, seed=order)
+        descs = [d for d in descs if d[1] - (1 if d[0] == zf.KIND_RFFT else 0) == order]
+        times = {}
+        outs = {}
+        for logr in (3, 4):
+            if not _legal(order, logr):
+                continue
+            table = list(base)
+            table[order] = logr
+            rb = zf.RaggedBatch(descs, logr=table)
+            times[logr] = workloads.time_ms(torch, rb.run)
+            torch.cuda.synchronize()
+            outs[logr] = _outputs(torch, descs)
+            rb.close()
+        if len(outs) == 2:
+            worst = max(_rel(a, b) for a, b in zip(outs[3], outs[4]))
+            if worst > 1e-6:
+                raise RuntimeError(f"order {order}: the two configurations disagree (rel-L2 {worst:.3e})")
+        best = min(times, key=times.get)
+        results[order] = (best, times)
+        if verbose:
+            print(f"order {order:2d}: " + ", ".join(f"{1 << k:2d} pts/thread {v:.4f} ms" for k, v in sorted(times.items()))
+                  + f" -> {1 << best}", flush=True)
+    props = torch.cuda.get_device_properties(0)
+    with open(out_path, "w") as f:
+        f.write("# zlfft tuning table, written by python -m zldsp_fft_amd.autotune\n")
+        f.write(f"# device: {props.name} ({getattr(props, 'gcnArchName', '?')}), {props.multi_processor_count} CUs\n")
+        f.write(f"# batch per order: {points} points cycling through every transform variant\n")
+        f.write("# key order value   (ragged_logr: log2 points per thread of the ragged kernel)\n")
+        for order, (best, times) in sorted(results.items()):
+            detail = " ".join(f"{1 << k}pts={v:.4f}ms" for k, v in sorted(times.items()))
+            f.write(f"ragged_logr {order} {best}   # {detail}\n")
+    return results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "zlfft_tune.txt"))
+    ap.add_argument("--points", type=int, default=1 << 24)
+    args = ap.parse_args()
+    tune(args.points, args.out)
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

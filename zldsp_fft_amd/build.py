@@ -51,6 +51,8 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     for l in range(MIN_MACRO_LOGL, MAX_MACRO_LOGL + 1):
         units.append((os.path.join(CSRC, "zlfft_macro.hip"), os.path.join(OBJ, f"macro{l}.o"), [f"-DZL_LOGL={l}"]))
     units.append((os.path.join(CSRC, "zlfft_mfma.hip"), os.path.join(OBJ, "mfma.o"), []))
+    units.append((os.path.join(CSRC, "zlfft_ragged_k.hip"), os.path.join(OBJ, "ragged_k.o"), []))
+    units.append((os.path.join(CSRC, "zlfft_ragged_api.hip"), os.path.join(OBJ, "ragged_api.o"), []))
     units.append((os.path.join(CSRC, "zlfft_api.hip"), os.path.join(OBJ, "api.o"), []))
     cmds = [[HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:

@@ -3,7 +3,10 @@
 // synchronous single-transform calls used by the drop-in C++ classes.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -19,6 +22,59 @@ int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+// shared with the other C-ABI translation units
+int zl_fail(int code, const std::string& msg) { return fail(code, msg); }
+
+// ---- tuning table -------------------------------------------------------
+// Text file of "<key> <order> <value>" lines written by
+// zldsp_fft_amd/autotune.py: $ZLFFT_TUNE_FILE, else zlfft_tune.txt next to
+// this library.  Missing entries keep the built-in defaults.
+namespace {
+struct Tune {
+    int ragged_logr[zl::kRaggedMaxOrderTune + 1];
+};
+Tune g_tune;
+std::once_flag g_tune_once;
+
+void load_tune() {
+    for (int n = 0; n <= zl::kRaggedMaxOrderTune; ++n) g_tune.ragged_logr[n] = n <= 3 ? 3 : 4;
+    std::string path;
+    if (const char* env = getenv("ZLFFT_TUNE_FILE")) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr(reinterpret_cast<void*>(&zlfft_version), &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/zlfft_tune.txt";
+        }
+    }
+    FILE* f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+    if (!f) return;
+    char key[64];
+    int order = 0, value = 0;
+    char line[256];
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#') continue;
+        if (sscanf(line, "%63s %d %d", key, &order, &value) != 3) continue;
+        if (!strcmp(key, "ragged_logr") && order >= 0 && order <= zl::kRaggedMaxOrderTune &&
+            (value == 3 || value == 4))
+            g_tune.ragged_logr[order] = value;
+    }
+    fclose(f);
+}
+}  // namespace
+
+int zl_tuned_ragged_logr(int order) {
+    std::call_once(g_tune_once, load_tune);
+    if (order < 0 || order > zl::kRaggedMaxOrderTune) return 4;
+    return g_tune.ragged_logr[order];
+}
+
+namespace {
 
 int hip_fail(hipError_t e, const char* what) {
     return fail(ZLFFT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));

@@ -64,6 +64,7 @@ KernelEntry make_entry() {
 constexpr int kMinMacroLogL = 6;
 constexpr int kMaxMacroLogL = 10;
 constexpr int kMaxOrder = 28;  // largest CFFT order
+constexpr int kRaggedMaxOrderTune = 14;  // orders of the ragged tuning table
 
 // columns (butterflies) per workgroup of a radix-2^logl HBM pass
 constexpr int cfg_macro_c(int logl) { return logl <= 8 ? 32 : 16; }
@@ -128,6 +129,10 @@ MacroEntry make_macro_entry() {
 
 // MFMA split-precision 4096-point kernel for a precision code, or null
 void* zl_mfma_kernel(int prec, int backward);
+
+// ragged-batch kernels (zlfft_ragged_k.hip)
+void* zl_ragged_kernel();
+void* zl_ragged_reset_kernel();
 
 #define ZL_DECLARE_MACRO(n) void zl_fill_macro_##n(zl::MacroTable& tab);
 ZL_DECLARE_MACRO(6)
