@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA split-precision variants")
+    ap.add_argument("--no-configs", action="store_true", help="skip the analyzer and large-order lines")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
@@ -136,6 +137,100 @@ def ragged_section(args) -> dict:
     return {"workload": "ragged_10000_orders6-14_mixed", "bytes_per_launch": nbytes, "ms_per_launch": round(ms, 5),
             "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
             "transforms_per_s": round(len(descs) / (ms * 1e-3), 1)}
+
+
+ROTATE = 5  # buffer sets per line: > 256 MiB in total, so the MALL cannot hold the working set
+
+
+def _timed(torch, stream, fns, reps, warmup=5):
+    """Mean ms per call over `reps` calls cycling through `fns`, HIP events on `stream`."""
+    for i in range(warmup):
+        fns[i % len(fns)]()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for i in range(reps):
+        fns[i % len(fns)]()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _line(nbytes, flops, count, ms, rel=None):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    d = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+         "transforms_per_s": round(count / (ms * 1e-3), 1), "gflops": round(flops / (ms * 1e-3) / 1e9, 1)}
+    if rel is not None:
+        d["rel_l2_vs_fp64"] = rel
+    return d
+
+
+def config_lines(args) -> dict:
+    """Throughput of the analyzer batch (4096 RFFTs of 2048 reals: AoS bins and
+    squared magnitudes) and of the large SoA CFFTs (orders 16/18/20/22 at
+    batch 64/16/4/1, forward and backward).  Each line cycles through
+    ROTATE buffer sets; GB/s counts one read of the inputs and one write of
+    the outputs; GFLOP/s uses 5 N log2 N per CFFT and 2.5 N log2 N per RFFT.
+    rocFFT (through torch.fft) is timed beside it as an external reference
+    point on the AoS layouts it supports."""
+    import torch
+    import oracle
+    import zldsp_fft_amd as zf
+    from zldsp_fft_amd import signals
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    reps = max(20, args.steps // 4)
+    out = {}
+    # analyzer batch: 4096 Hann-windowed frames of 2048 reals
+    n, frames = 2048, 4096
+    nb = n // 2 + 1
+    plan = zf.RFFT(11)
+    x0 = torch.from_numpy(signals.analyzer_frames(2, frames, n).astype(np.float32)).to(dev)
+    xs = [x0] + [x0.clone() for _ in range(ROTATE - 1)]  # distinct addresses, same frames
+    ys = [torch.empty(frames * nb, dtype=torch.complex64, device=dev) for _ in range(ROTATE)]
+    ps = [torch.empty(frames * nb, dtype=torch.float32, device=dev) for _ in range(ROTATE)]
+    flops = 2.5 * n * 11 * frames
+    ms = _timed(torch, stream, [lambda i=i: plan.forward(xs[i].view(-1), ys[i]) for i in range(ROTATE)], reps)
+    ref = oracle.rfft_forward(xs[0][7].cpu().numpy().astype(np.float64), "f64")
+    got = ys[0].view(frames, nb)[7].cpu().numpy()
+    out["rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, ms,
+                                       float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    ms = _timed(torch, stream, [lambda i=i: plan.forward_sqr_mag(xs[i].view(-1), ps[i]) for i in range(ROTATE)], reps)
+    got = ps[0].view(frames, nb)[7].cpu().numpy()
+    out["rfft2048_b4096_sqr_mag"] = _line(frames * (4 * n + 4 * nb), flops, frames, ms,
+                                           float(np.linalg.norm(got - np.abs(ref) ** 2) / np.linalg.norm(np.abs(ref) ** 2)))
+    ms = _timed(torch, stream, [lambda i=i: torch.fft.rfft(xs[i], dim=-1) for i in range(ROTATE)], reps)
+    out["rocfft_rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, ms)
+    plan.close()
+    del xs, ys, ps
+    torch.cuda.empty_cache()
+    # large SoA CFFTs: 4,194,304 points per call
+    for order, batch in ((16, 64), (18, 16), (20, 4), (22, 1)):
+        n = 1 << order
+        plan = zf.CFFT(order)
+        g = torch.Generator(device=dev).manual_seed(3)
+        ins = [(torch.randn(batch * n, device=dev, generator=g), torch.randn(batch * n, device=dev, generator=g))
+               for _ in range(ROTATE)]
+        outs = [(torch.empty(batch * n, device=dev), torch.empty(batch * n, device=dev)) for _ in range(ROTATE)]
+        for name, fn in (("forward", plan.forward), ("backward", plan.backward)):
+            ms = _timed(torch, stream, [lambda i=i, fn=fn: fn(ins[i], outs[i]) for i in range(ROTATE)], reps)
+            x = ins[0][0][:n].cpu().numpy().astype(np.float64) + 1j * ins[0][1][:n].cpu().numpy()
+            ref = oracle.cfft(x, name == "forward", "f64")
+            got = outs[0][0][:n].cpu().numpy() + 1j * outs[0][1][:n].cpu().numpy().astype(np.float64)
+            out[f"cfft_o{order}_b{batch}_soa_{name}"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, ms,
+                                                              float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+        plan.close()
+        del ins, outs
+        torch.cuda.empty_cache()
+    return out
+
+
+def rocfft_c4(x, stream, args) -> dict:
+    """rocFFT (through torch.fft) on the same 16384 x 4096 AoS batch: an
+    external reference point, timed the same way."""
+    import torch
+    ms = _timed(torch, stream, [lambda: torch.fft.fft(x, dim=-1)], max(20, args.steps // 4), warmup=10)
+    return _line(BYTES_PER_STEP, 5.0 * N * ORDER * BATCH, BATCH, ms)
 
 
 def traffic_bytes():
@@ -238,6 +333,9 @@ def main():
     }
     if not args.no_mfma:
         out["mfma"] = mfma_variants(x, y, xs, ref, stream, lib, args)
+    out["rocfft"] = rocfft_c4(x, stream, args)
+    if not args.no_configs:
+        out["configs"] = config_lines(args)
     if not args.no_ragged:
         del x, y
         torch.cuda.empty_cache()

@@ -75,3 +75,13 @@ def test_python_mirror_validates_order():
     with pytest.raises(zf.ZlfftError) as e:
         zf.RFFT(0)
     assert e.value.code == 1
+
+
+def test_ragged_desc_numpy_image_matches_struct():
+    """RAGGED_DESC_DTYPE (bulk descriptor tables) has the layout of
+    zlfft_ragged_desc: same size and field offsets as the ctypes mirror."""
+    import ctypes
+    d = zf.RAGGED_DESC_DTYPE
+    assert d.itemsize == ctypes.sizeof(zf.RaggedDesc) == 48
+    for name, _ in zf.RaggedDesc._fields_:
+        assert d.fields[name][1] == getattr(zf.RaggedDesc, name).offset, name

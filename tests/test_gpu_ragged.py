@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 import zldsp_fft_amd as zf
-from zldsp_fft_amd import signals, workloads
+from zldsp_fft_amd import workloads
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -116,34 +116,39 @@ def test_ragged_rejects_bad_descriptors():
     assert e.value.code == 5
 
 
+def _oracle(kind, direction, order, x):
+    if kind == zf.KIND_CFFT:
+        return oracle.cfft(x.astype(np.complex128), direction == zf.FORWARD, "f64")
+    if direction == zf.FORWARD:
+        return oracle.rfft_forward(x.astype(np.float64), "f64")
+    if direction == zf.FORWARD_SQR_MAG:
+        return np.abs(oracle.rfft_forward(x.astype(np.float64), "f64")) ** 2
+    return oracle.rfft_backward(x.astype(np.complex128), 1 << order, "f64")
+
+
 def test_ragged_points_per_thread_choices_agree():
     """Both points-per-thread configurations the autotuner chooses between
-    give the same results to 1e-6 on every order they support."""
-    for order in (3, 6, 9, 12, 13):
-        descs = workloads.ragged_uniform_order(order, 9, seed=order)
+    give the same results (per transform, rel-L2 1e-6) on every order they
+    support, and match the FP64 oracle; the descriptor table is built in
+    bulk (RaggedBatch.from_table) over one buffer per variant."""
+    for order in range(3, 15):
+        ut = workloads.UniformTable(order, 4 * len(workloads.VARIANTS), seed=order)
         res = {}
         for logr in (3, 4):
-            if logr == 4 and order < 4:
+            if (logr == 4 and order < 4) or (logr == 3 and order > 13):
                 continue
             table = [3 if n <= 3 else 4 for n in range(15)]
             table[order] = logr  # also the CFFT order of the RFFT entries (real size 2^(order + 1))
-            rb = zf.RaggedBatch(descs, logr=table)
+            rb = zf.RaggedBatch.from_table(ut.table, keep=ut, logr=table)
             rb.run()
             torch.cuda.synchronize()
-            res[logr] = [as_np(d[5]) for d in descs]
+            res[logr] = [o.cpu().numpy().astype(np.float64) for o in ut.outputs()]
             rb.close()
         if len(res) == 2:
             for a, b in zip(res[3], res[4]):
-                assert np.linalg.norm(a - b) / np.linalg.norm(b) <= TOL, order
-        for d, got in zip(descs, res[min(res)]):
-            kind, o, direction, layout, inb, outb = d
-            x = as_np(inb)
-            if kind == zf.KIND_CFFT:
-                ref = oracle.cfft(x.astype(np.complex128), direction == zf.FORWARD, "f64")
-            elif direction == zf.FORWARD:
-                ref = oracle.rfft_forward(x.astype(np.float64), "f64")
-            elif direction == zf.FORWARD_SQR_MAG:
-                ref = np.abs(oracle.rfft_forward(x.astype(np.float64), "f64")) ** 2
-            else:
-                ref = oracle.rfft_backward(x.astype(np.complex128), 1 << o, "f64")
-            assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 2 * TOL, (order, d[:4])
+                e = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
+                assert e.max() <= TOL, order
+        for v in range(len(ut.bufs)):
+            (kind, direction, layout, o), x, got = ut.transform(v, 3)
+            ref = _oracle(kind, direction, o, x)
+            assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 2 * TOL, (order, kind, direction, layout)

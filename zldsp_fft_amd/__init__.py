@@ -63,6 +63,13 @@ class RaggedDesc(ctypes.Structure):
                 ("out0", ctypes.c_void_p), ("out1", ctypes.c_void_p)]
 
 
+# numpy image of zlfft_ragged_desc (48 bytes), for building large tables without
+# one Python object per transform
+RAGGED_DESC_DTYPE = np.dtype([("kind", "<i4"), ("order", "<i4"), ("direction", "<i4"), ("layout", "<i4"),
+                              ("in0", "<u8"), ("in1", "<u8"), ("out0", "<u8"), ("out1", "<u8")])
+assert RAGGED_DESC_DTYPE.itemsize == ctypes.sizeof(RaggedDesc)
+
+
 class ZlfftError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"zlfft error {code} ({_ERRNAMES.get(code, '?')}): {msg}")
@@ -354,24 +361,35 @@ class RaggedBatch:
     def __init__(self, descs, logr=None):
         """``logr``: optional points per thread (log2, 3 or 4) for CFFT orders
         0..14 instead of the tuned table."""
-        self._lib = load_library()
-        arr = (RaggedDesc * max(1, len(descs)))()
-        self._keep = []
+        table = np.zeros(len(descs), dtype=RAGGED_DESC_DTYPE)
         for i, (kind, order, direction, layout, inb, outb) in enumerate(descs):
             def ptrs(b):
                 if isinstance(b, (tuple, list)):
                     return _ptr(b[0]), _ptr(b[1])
-                return _ptr(b), None
+                return _ptr(b), 0
             i0, i1 = ptrs(inb)
             o0, o1 = ptrs(outb)
-            arr[i] = RaggedDesc(kind, order, direction, layout, i0, i1, o0, o1)
-            self._keep.append((inb, outb))
+            table[i] = (kind, order, direction, layout, i0, i1, o0, o1)
+        self._create(table, [(d[4], d[5]) for d in descs], logr)
+
+    @classmethod
+    def from_table(cls, table: np.ndarray, keep=(), logr=None) -> "RaggedBatch":
+        """A batch from a RAGGED_DESC_DTYPE array of device addresses; `keep`
+        holds whatever owns that memory for the lifetime of the batch."""
+        self = cls.__new__(cls)
+        self._create(np.ascontiguousarray(table, dtype=RAGGED_DESC_DTYPE), keep, logr)
+        return self
+
+    def _create(self, table, keep, logr):
+        self._lib = load_library()
+        self._keep = keep
         h = ctypes.c_void_p()
+        arr = table.ctypes.data_as(ctypes.POINTER(RaggedDesc))
         if logr is None:
-            _check(self._lib.zlfft_ragged_create(ctypes.byref(h), arr, len(descs)))
+            _check(self._lib.zlfft_ragged_create(ctypes.byref(h), arr, len(table)))
         else:
             lr = (ctypes.c_int * 15)(*logr)
-            _check(self._lib.zlfft_ragged_create_config(ctypes.byref(h), arr, len(descs), lr))
+            _check(self._lib.zlfft_ragged_create_config(ctypes.byref(h), arr, len(table), lr))
         self._h = h
 
     def run(self, stream: int | None = None) -> None:

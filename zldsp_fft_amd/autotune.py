@@ -16,8 +16,6 @@ import argparse
 import os
 import sys
 
-import numpy as np
-
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 
@@ -29,23 +27,12 @@ def _legal(order: int, logr: int) -> bool:
     return (logr == 4 and order >= 4) or (logr == 3 and order >= 3 and (1 << (order - 3)) <= 1024)
 
 
-def _outputs(torch, descs):
-    res = []
-    for d in descs:
-        o = d[5]
-        res.append(tuple(t.clone() for t in o) if isinstance(o, tuple) else o.clone())
-    return res
-
-
 def _rel(a, b):
+    """Largest per-transform rel-L2 between two (transforms, floats) output buffers."""
     import torch
-    if isinstance(a, tuple):
-        num = sum(torch.linalg.vector_norm((x - y).double()) ** 2 for x, y in zip(a, b)) ** 0.5
-        den = sum(torch.linalg.vector_norm(y.double()) ** 2 for y in b) ** 0.5
-    else:
-        num = torch.linalg.vector_norm((a - b).to(torch.complex128) if a.is_complex() else (a - b).double())
-        den = torch.linalg.vector_norm(b.to(torch.complex128) if b.is_complex() else b.double())
-    return float(num / den)
+    num = torch.linalg.vector_norm((a - b).double(), dim=1)
+    den = torch.linalg.vector_norm(b.double(), dim=1).clamp_min(1e-300)
+    return float((num / den).max())
 
 
 def tune(points: int, out_path: str, verbose: bool = True) -> dict:
@@ -53,9 +40,8 @@ def tune(points: int, out_path: str, verbose: bool = True) -> dict:
     results = {}
     base = [3 if n <= 3 else 4 for n in range(15)]
     for order in range(3, 15):
-        count = max(9, points >> order)
-        descs = workloads.ragged_uniform_order(order, count, seed=order)
-        descs = [d for d in descs if d[1] - (1 if d[0] == zf.KIND_RFFT else 0) == order]
+        count = max(len(workloads.VARIANTS), points >> order)
+        ut = workloads.UniformTable(order, count, seed=order)
         times = {}
         outs = {}
         for logr in (3, 4):
@@ -63,15 +49,17 @@ def tune(points: int, out_path: str, verbose: bool = True) -> dict:
                 continue
             table = list(base)
             table[order] = logr
-            rb = zf.RaggedBatch(descs, logr=table)
+            rb = zf.RaggedBatch.from_table(ut.table, keep=ut, logr=table)
             times[logr] = workloads.time_ms(torch, rb.run)
             torch.cuda.synchronize()
-            outs[logr] = _outputs(torch, descs)
+            outs[logr] = [o.clone() for o in ut.outputs()]
             rb.close()
         if len(outs) == 2:
             worst = max(_rel(a, b) for a, b in zip(outs[3], outs[4]))
             if worst > 1e-6:
                 raise RuntimeError(f"order {order}: the two configurations disagree (rel-L2 {worst:.3e})")
+        del ut, outs
+        torch.cuda.empty_cache()
         best = min(times, key=times.get)
         results[order] = (best, times)
         if verbose:

@@ -56,21 +56,82 @@ def ragged_mix(count: int, seed: int, orders=(6, 14)):
     return out
 
 
-def ragged_uniform_order(order: int, count: int, seed: int):
-    """count transforms of one size, cycling through every variant (CFFT
-    forward/backward x AoS/SoA, RFFT forward/backward x AoS/SoA, sqr-mag);
-    `order` is the real size for the RFFT entries, so all share one CFFT order
-    only for the CFFT ones."""
-    import torch
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    variants = [(zf.KIND_CFFT, d, l) for d in (zf.FORWARD, zf.BACKWARD) for l in (zf.AOS, zf.SOA)]
-    variants += [(zf.KIND_RFFT, d, l) for d in (zf.FORWARD, zf.BACKWARD) for l in (zf.AOS, zf.SOA)]
-    variants += [(zf.KIND_RFFT, zf.FORWARD_SQR_MAG, zf.AOS)]
-    out = []
-    for i in range(count):
-        kind, d, l = variants[i % len(variants)]
-        out.append(ragged_desc(torch, g, kind, order + (1 if kind == zf.KIND_RFFT else 0), d, l))
-    return out
+VARIANTS = ([(zf.KIND_CFFT, d, l) for d in (zf.FORWARD, zf.BACKWARD) for l in (zf.AOS, zf.SOA)]
+            + [(zf.KIND_RFFT, d, l) for d in (zf.FORWARD, zf.BACKWARD) for l in (zf.AOS, zf.SOA)]
+            + [(zf.KIND_RFFT, zf.FORWARD_SQR_MAG, zf.AOS)])
+
+
+def _side_floats(kind, direction, layout, order, side):
+    """Floats per transform of one buffer of one side (`side` 0: in, 1: out)
+    and how many such buffers (2 for SoA complex sides)."""
+    n = 1 << order
+    nb = n // 2 + 1
+    if kind == zf.KIND_CFFT:
+        return 2 * n if layout == zf.AOS else n, 1 if layout == zf.AOS else 2
+    real_side = (side == 0) == (direction != zf.BACKWARD)
+    if real_side:
+        return n, 1
+    if direction == zf.FORWARD_SQR_MAG:
+        return nb, 1
+    return (2 * nb, 1) if layout == zf.AOS else (nb, 2)
+
+
+class UniformTable:
+    """count transforms of one CFFT order (RFFT entries have real order + 1)
+    cycling through every variant, as a descriptor table over one input and
+    one output buffer per variant (U[-1, 1) inputs): no per-transform objects."""
+
+    def __init__(self, order: int, count: int, seed: int):
+        import torch
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        self.table = np.zeros(count, dtype=zf.RAGGED_DESC_DTYPE)
+        self.bufs = []
+        idx = np.arange(count)
+        for v, (kind, d, l) in enumerate(VARIANTS):
+            rows = idx[idx % len(VARIANTS) == v]
+            m = len(rows)
+            if m == 0:
+                continue
+            o = order + (1 if kind == zf.KIND_RFFT else 0)
+            fi, ki = _side_floats(kind, d, l, o, 0)
+            fo, ko = _side_floats(kind, d, l, o, 1)
+            ins = [torch.rand(m * fi, generator=g, device="cuda") * 2 - 1 for _ in range(ki)]
+            outs = [torch.empty(m * fo, device="cuda") for _ in range(ko)]
+            self.bufs.append(((kind, d, l, o, fo), ins, outs))
+            k = np.arange(m, dtype=np.uint64)
+            t = self.table[rows]
+            t["kind"], t["order"], t["direction"], t["layout"] = kind, o, d, l
+            t["in0"] = ins[0].data_ptr() + 4 * fi * k
+            t["in1"] = ins[1].data_ptr() + 4 * fi * k if ki == 2 else 0
+            t["out0"] = outs[0].data_ptr() + 4 * fo * k
+            t["out1"] = outs[1].data_ptr() + 4 * fo * k if ko == 2 else 0
+            self.table[rows] = t
+
+    def outputs(self):
+        """Each output buffer as a (transforms, floats per transform) view."""
+        return [o.view(-1, meta[4]) for meta, _, outs in self.bufs for o in outs]
+
+    def transform(self, v: int, i: int):
+        """(kind, direction, layout, order), input and output of transform i
+        of variant v, as numpy (complex sides as complex128)."""
+        (kind, d, l, o, fo), ins, outs = self.bufs[v]
+
+        def side(bufs, per, complex_side):
+            rows = [b.view(-1, per)[i].cpu().numpy().astype(np.float64) for b in bufs]
+            if not complex_side:
+                return rows[0]
+            if len(rows) == 2:
+                return rows[0] + 1j * rows[1]
+            return rows[0][0::2] + 1j * rows[0][1::2]
+
+        m = outs[0].numel() // fo
+        fi = ins[0].numel() // m
+        cin = kind == zf.KIND_CFFT or d == zf.BACKWARD
+        cout = kind == zf.KIND_CFFT or d == zf.FORWARD
+        return (kind, d, l, o), side(ins, fi, cin), side(outs, fo, cout)
+
+    def nbytes(self) -> int:
+        return sum(4 * t.numel() for _, ins, outs in self.bufs for t in ins + outs)
 
 
 def algorithmic_bytes(desc) -> int:
