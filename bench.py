@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA split-precision variants")
     ap.add_argument("--no-configs", action="store_true", help="skip the analyzer and large-order lines")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch line")
+    ap.add_argument("--only", choices=("main", "mfma", "configs", "ragged"), default=None,
+                    help="run one section alone (for profiler runs): the headline batch, the MFMA variants, "
+                         "the analyzer / large-order lines or the ragged batch")
+    ap.add_argument("--mfma-precisions", default=None,
+                    help="comma-separated subset of the MFMA variant names")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
     return ap.parse_args()
@@ -91,7 +96,10 @@ def mfma_variants(x, y, xs, ref, stream, lib, args) -> dict:
     res = {}
     idx = list(range(0, BATCH, BATCH // 16))
     steps = args.steps
+    wanted = args.mfma_precisions.split(",") if args.mfma_precisions else None
     for name, prec in MFMA_VARIANTS:
+        if wanted is not None and name not in wanted:
+            continue
         plan = zf.CFFT(ORDER, prec)
         h, sp = plan._h, stream.cuda_stream
 
@@ -260,6 +268,11 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.only in ("configs", "ragged"):
+        out = {"section": args.only}
+        out[args.only] = config_lines(args) if args.only == "configs" else ragged_section(args)
+        print(json.dumps(out), flush=True)
+        return
     dev = torch.device("cuda:0")
     plan = zf.CFFT(ORDER)
     g = torch.Generator(device=dev).manual_seed(4)
@@ -279,6 +292,12 @@ def main():
         for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
+    if args.only == "mfma":
+        idx = list(range(0, BATCH, BATCH // 16))
+        xs = x[idx].cpu().numpy().astype(np.complex128)
+        ref = np.stack([oracle.cfft(r, True, "f64") for r in xs])
+        print(json.dumps({"section": "mfma", "mfma": mfma_variants(x, y, xs, ref, stream, lib, args)}), flush=True)
+        return
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -331,9 +350,12 @@ def main():
         "gflops_5nlog2n": round(5 * N * ORDER * BATCH / (ms_per_step * 1e-3) / 1e9, 1),
         "kernel_ms_avg": round(kern_ms, 5),
     }
+    if args.only == "main":
+        args.no_mfma = args.no_configs = args.no_ragged = args.no_cpu = True
     if not args.no_mfma:
         out["mfma"] = mfma_variants(x, y, xs, ref, stream, lib, args)
-    out["rocfft"] = rocfft_c4(x, stream, args)
+    if args.only != "main":
+        out["rocfft"] = rocfft_c4(x, stream, args)
     if not args.no_configs:
         out["configs"] = config_lines(args)
     if not args.no_ragged:

@@ -31,7 +31,8 @@ import numpy as np
 __all__ = ["CFFT", "RFFT", "ZlfftError", "lib_path", "load_library", "AOS", "SOA"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-lib_path = os.path.join(_HERE, "libzlfft.so")
+# the in-package build; ZLFFT_LIB names another build of the same library
+lib_path = os.environ.get("ZLFFT_LIB") or os.path.join(_HERE, "libzlfft.so")
 
 # constants of include/zlfft.h
 OK = 0

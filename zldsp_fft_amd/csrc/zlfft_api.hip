@@ -149,9 +149,8 @@ __global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi
     out[idx] = make_float2((float)cs, (float)sn);
 }
 
-// FP16 MFMA kernels' twiddle image: every twiddle W of the image divided by
-// 16 (exact), i.e. the direct / low-factor entries [0, lo_n) of one pass
-// scaled and the high-factor entries [lo_n, lo_n + hi_n) copied.
+// FP16 MFMA kernels' twiddle table: entries [0, lo_n) divided by 16 (exact),
+// entries [lo_n, lo_n + hi_n) copied.
 __global__ void scale_twiddles16(const float2* in, float2* out, int lo_n, int hi_n) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= lo_n + hi_n) return;
@@ -194,9 +193,10 @@ struct zlfft_plan_s {
     float2* d_tw = nullptr;
     float2* d_img = nullptr;    // LDS twiddle image for the streaming kernels
     float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
-    float2* d_img16 = nullptr;  // twiddle image / 16 for the FP16 MFMA kernels
+    float2* d_tw16 = nullptr;   // pass twiddles / 16 for the FP16 MFMA kernels
     int cus = 0;                // compute units of the device
     int stream_blocks[4][2][2] = {};  // resident workgroups per CU of each streaming kernel
+    int mfma_blocks = 0;              // resident workgroups per CU of the persistent MFMA kernel
     float2* d_rtw = nullptr;
     int n_rtw = 0;
     // multi-pass transforms (corder > kMaxLdsOrder)
@@ -322,7 +322,10 @@ int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, si
                 void* stream) {
     if (lin != ZLFFT_AOS || lout != ZLFFT_AOS)
         return fail(ZLFFT_ERR_UNSUPPORTED, "the MFMA precisions take AoS input and output");
-    void* fn = zl_mfma_kernel(p->precision, backward);
+    // more transforms than resident workgroups: the persistent form
+    const long long resident = (long long)p->mfma_blocks * p->cus;
+    const bool streaming = resident > 0 && p->d_sink && (long long)batch > resident;
+    void* fn = zl_mfma_kernel(p->precision, backward, streaming ? 1 : 0);
     if (!fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no MFMA kernel for this precision");
     if (batch == 0) return ZLFFT_OK;
     if (batch > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
@@ -330,10 +333,12 @@ int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, si
     a.in0 = in0;
     a.out0 = out0;
     const bool f16 = p->precision == ZLFFT_PREC_F16_MFMA || p->precision == ZLFFT_PREC_F16_MFMA_SPLIT;
-    a.tw = f16 ? p->d_img16 : p->d_img;
+    a.tw = f16 ? p->d_tw16 : p->d_tw;
+    a.sink = p->d_sink;
     a.batch = (long long)batch;
     void* params[] = {&a};
-    ZL_HIP(hipLaunchKernel(fn, dim3((unsigned)batch), dim3(256), params, 0, reinterpret_cast<hipStream_t>(stream)));
+    const unsigned grid = (unsigned)(streaming ? resident : (long long)batch);
+    ZL_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), params, 0, reinterpret_cast<hipStream_t>(stream)));
     return ZLFFT_OK;
 }
 
@@ -523,6 +528,10 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                     if (e != hipSuccess) return cleanup(hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
                     p->stream_blocks[k][i][o] = nb;
                 }
+        if (precision != ZLFFT_PREC_F32) {
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p->mfma_blocks, zl_mfma_kernel(precision, 0, 1), 256, 0);
+            if (e != hipSuccess) return cleanup(hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
+        }
         const zl::KernelEntry& ke = (*p->tab)[zl::C2C][0][0];
         if (ke.stream_fn[0] || (*p->tab)[zl::R2C][0][0].stream_fn[0]) {
             e = hipMalloc(&p->d_sink, (size_t)zl::kSinkBytes);
@@ -538,17 +547,6 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                                                                               ke.img_lo[q], ke.img_hi[q]);
                 off += (size_t)cnt;
             }
-            if (precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT) {
-                e = hipMalloc(&p->d_img16, sizeof(float2) * (size_t)ke.img_total);
-                if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
-                size_t o16 = 0;
-                for (int q = 1; q < ke.np; ++q) {
-                    const int cnt = ke.img_lo[q] + ke.img_hi[q];
-                    scale_twiddles16<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_img + o16, p->d_img16 + o16,
-                                                                               ke.img_lo[q], ke.img_hi[q]);
-                    o16 += (size_t)cnt;
-                }
-            }
         }
     }
     const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[zl::C2C][0][0];
@@ -560,6 +558,11 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
             const int cnt = ((1 << ent.lr[q]) - 1) << ent.lw[q];
             gen_pass_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_tw + off, ent.lr[q], ent.lw[q], cnt);
             off += (size_t)cnt;
+        }
+        if (precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT) {
+            e = hipMalloc(&p->d_tw16, sizeof(float2) * (size_t)ent.tw_total);
+            if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
+            scale_twiddles16<<<(ent.tw_total + 255) / 256, 256, 0, p->stream>>>(p->d_tw, p->d_tw16, ent.tw_total, 0);
         }
         e = hipGetLastError();
         if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
@@ -587,7 +590,7 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (p->d_tw) (void)hipFree(p->d_tw);
     if (p->d_img) (void)hipFree(p->d_img);
     if (p->d_sink) (void)hipFree(p->d_sink);
-    if (p->d_img16) (void)hipFree(p->d_img16);
+    if (p->d_tw16) (void)hipFree(p->d_tw16);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
     if (p->d_stage) (void)hipFree(p->d_stage);
     if (p->d_work) (void)hipFree(p->d_work);

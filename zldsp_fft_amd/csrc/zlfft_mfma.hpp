@@ -2,7 +2,7 @@
 // pass is a 16x16 complex DFT-matrix product on FP16 or BF16 MFMA with FP32
 // accumulate, with the operands optionally split into 2 (hi + lo) or 3
 // (hi + mid + lo) parts to win back precision.  Twiddles between the passes
-// stay in FP32 (VALU), from the same LDS image as fft_stream_kernel.
+// stay in FP32 (VALU), from an LDS copy of the plan's pass twiddles.
 //
 // v_mfma_f32_16x16x32_{f16,bf16} lane layout (checked on gfx950):
 //   A lane l holds A[l % 16][8 (l / 16) + e], e < 8
@@ -20,6 +20,10 @@
 #include "zlfft_device.hpp"
 
 namespace zl {
+
+#ifndef ZL_MFMA_STREAM_WAVES
+#define ZL_MFMA_STREAM_WAVES 2
+#endif
 
 enum MfmaPrec : int { MF_F16 = 0, MF_BF16 = 1 };
 
@@ -58,25 +62,52 @@ __device__ __forceinline__ float bf16_trunc(float x) {
     return __builtin_bit_cast(float, __builtin_bit_cast(unsigned, x) & 0xffff0000u);
 }
 
+typedef float zl_fv2 __attribute__((ext_vector_type(2)));
+
+// FP16 split of data: hi = v with the low 13 mantissa bits cleared (v
+// truncated to 11 significant bits, an FP16 value when v is in FP16's normal
+// range, so its conversion is exact) and lo = v - hi (exact), rounded to FP16
+// by packed conversions.  Integer and plain FP32 operations only: the
+// mixed-precision FMAs (v_fma_mix*) of the round-to-nearest form measured
+// slower on gfx950.  Unsplit (NS = 1): v rounded to nearest.
+template <int NS>
+__device__ __forceinline__ void split8_mask(const float* v, zl_h8 (&parts)[NS]) {
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        const float h0 = __uint_as_float(__float_as_uint(v[e]) & 0xffffe000u);
+        const float h1 = __uint_as_float(__float_as_uint(v[e + 1]) & 0xffffe000u);
+        const zl_h2 hp = __builtin_convertvector((zl_fv2){NS == 2 ? h0 : v[e], NS == 2 ? h1 : v[e + 1]}, zl_h2);
+        parts[0][e] = hp.x;
+        parts[0][e + 1] = hp.y;
+        if constexpr (NS == 2) {
+            const zl_h2 lp = __builtin_convertvector((zl_fv2){v[e] - h0, v[e + 1] - h1}, zl_h2);
+            parts[1][e] = lp.x;
+            parts[1][e + 1] = lp.y;
+        }
+    }
+}
+
 // Splits 8 FP32 values into NS parts of the MFMA input type, v = sum of the
 // parts to about NS x (mantissa bits) bits.
-//  FP16: parts are round-to-nearest, each the FP16 rounding of the previous
+//  FP16: data (RNE = false) as split8_mask; constants (RNE = true) with
+//        round-to-nearest parts, each the FP16 rounding of the previous
 //        remainder (exact in FP32).
 //  BF16: with three parts, the first two are truncations of the remainder to
 //        its top 16 bits (exactly BF16 values, remainders exact in FP32) and
 //        the last is rounded to nearest; with two, both round to nearest.
-// neg1 must be -1.0f, but not a compile-time constant: with a constant the
-// compiler rewrites fma(h, -1, x) as x - h and loses the single
-// mixed-precision FMA (v_fma_mix) that computes the FP16 remainder.
-template <int P, int NS>
-__device__ __forceinline__ void split8(const float* v, typename MfOp<P>::V (&parts)[NS], float neg1) {
+template <int P, int NS, bool RNE = false>
+__device__ __forceinline__ void split8(const float* v, typename MfOp<P>::V (&parts)[NS]) {
     if constexpr (P == MF_F16) {
         static_assert(NS <= 2, "FP16 splits into at most two parts");
+        if constexpr (RNE) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const _Float16 h = (_Float16)v[e];
-            parts[0][e] = h;
-            if constexpr (NS == 2) parts[1][e] = (_Float16)__builtin_fmaf((float)h, neg1, v[e]);
+            for (int e = 0; e < 8; ++e) {
+                const _Float16 h = (_Float16)v[e];
+                parts[0][e] = h;
+                if constexpr (NS == 2) parts[1][e] = (_Float16)(v[e] - (float)h);
+            }
+        } else {
+            split8_mask<NS>(v, parts);
         }
     } else {
 #pragma unroll
@@ -104,6 +135,15 @@ __device__ __forceinline__ void split8(const float* v, typename MfOp<P>::V (&par
     }
 }
 
+// FP16 split of the scaled values s v (s a power of two, so s v is exact).
+template <int NS>
+__device__ __forceinline__ void split8_scaled(const float* v, float s, zl_h8 (&parts)[NS]) {
+    float w[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) w[e] = v[e] * s;
+    split8_mask<NS>(w, parts);
+}
+
 // cos(2 pi j / 16), j = 0..15, in double
 __device__ __forceinline__ double cos16(int j) {
     constexpr double c1 = 0.92387953251128675613, c2 = 0.70710678118654752440, c3 = 0.38268343236508977173;
@@ -113,8 +153,8 @@ __device__ __forceinline__ double cos16(int j) {
 
 // A operands (DFT-16 matrix, split in NS parts) of this lane
 template <int P, int NS>
-__device__ __forceinline__ void dft16_operands(int lane, typename MfOp<P>::V (&ar)[NS], typename MfOp<P>::V (&ai)[NS],
-                                               float neg1) {
+__device__ __forceinline__ void dft16_operands(int lane, typename MfOp<P>::V (&ar)[NS],
+                                               typename MfOp<P>::V (&ai)[NS]) {
     const int q = lane & 15, h = lane >> 4;
     float fr[8], fi[8];
 #pragma unroll
@@ -126,15 +166,17 @@ __device__ __forceinline__ void dft16_operands(int lane, typename MfOp<P>::V (&a
         fr[e] = (e & 1) ? (float)(-im) : (float)re;
         fi[e] = (e & 1) ? (float)re : (float)im;
     }
-    split8<P, NS>(fr, ar, neg1);
-    split8<P, NS>(fi, ai, neg1);
+    split8<P, NS, true>(fr, ar);
+    split8<P, NS, true>(fi, ai);
 }
 
 // Y = F X for one group of 16 butterflies; x: this lane's 4 complex inputs
 // (m = 4h + i), returns this lane's 4 complex outputs (q = 4h + r).
-template <int P, int NS>
+// SCALED (FP16 only): the inputs are multiplied by the power of two s first,
+// inside the split.
+template <int P, int NS, bool SCALED = false>
 __device__ __forceinline__ void dft16_mfma(const float2 (&x)[4], float2 (&y)[4], const typename MfOp<P>::V (&ar)[NS],
-                                           const typename MfOp<P>::V (&ai)[NS], float neg1) {
+                                           const typename MfOp<P>::V (&ai)[NS], float s = 1.0f) {
     using OP = MfOp<P>;
     float v[8];
 #pragma unroll
@@ -143,7 +185,12 @@ __device__ __forceinline__ void dft16_mfma(const float2 (&x)[4], float2 (&y)[4],
         v[2 * i + 1] = x[i].y;
     }
     typename OP::V bp[NS];
-    split8<P, NS>(v, bp, neg1);
+    if constexpr (SCALED) {
+        static_assert(P == MF_F16, "scaled split is FP16 only");
+        split8_scaled<NS>(v, s, bp);
+    } else {
+        split8<P, NS>(v, bp);
+    }
     zl_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
     // terms of total split order < NS, smallest first
 #pragma unroll
@@ -159,84 +206,87 @@ __device__ __forceinline__ void dft16_mfma(const float2 (&x)[4], float2 (&y)[4],
     for (int r = 0; r < 4; ++r) y[r] = make_float2(cr[r], ci[r]);
 }
 
-// W_{16 w}^{m k} from the LDS twiddle image of geometry G, pass P
-// (for FP16 the image holds W / 16: its direct and low-factor entries are
-// divided by 16; m = 0 gives 1 / 16 likewise)
+// W_{16 w}^{m k} of pass P of geometry G from the LDS copy of the plan's pass
+// twiddles ([tw_off(P) + (m - 1) w + k]); for FP16 the table holds W / 16
+// and m = 0 gives 1 / 16 likewise
 template <class G, int P, int PREC>
-__device__ __forceinline__ float2 img_twiddle(const float2* ltw, int m, int k) {
-    using TI = TwImage<G>;
+__device__ __forceinline__ float2 mf_twiddle(const float2* ltw, int m, int k) {
     constexpr int w = 1 << G::lw(P);
-    const int mm = m > 0 ? m - 1 : 0;
-    float2 t;
-    if constexpr (w <= 16) {
-        t = ltw[TI::off(P) + mm * w + k];
-    } else {
-        t = cmul(ltw[TI::off(P) + mm * 16 + (k & 15)], ltw[TI::off(P) + TI::lo_n(P) + mm * (w / 16) + (k >> 4)]);
-    }
+    const float2 t = ltw[G::tw_off(P) + (m > 0 ? m - 1 : 0) * w + k];
     return m > 0 ? t : make_float2(PREC == MF_F16 ? 0.0625f : 1.f, 0.f);
 }
 
-// 4096-point CFFT, AoS in and out, one transform per workgroup of 256 threads
-// (4 waves, each owning 4 groups of 16 butterflies per pass); twiddles from
-// the LDS image of the plan (see TwImage).
+// largest |component| of this wave's inputs into wmax[wave]: bit patterns
+// with the sign cleared compare as unsigned integers in the order of the
+// magnitudes (the swap of the backward direction does not change it)
+__device__ __forceinline__ void mfma_wave_absmax(const float2 (&x)[4][4], float* wmax) {
+    unsigned mb = 0u;
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi)
+            mb = max(mb, max(__float_as_uint(x[gg][mi].x) & 0x7fffffffu, __float_as_uint(x[gg][mi].y) & 0x7fffffffu));
+    float mx = __uint_as_float(mb);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+}
+
+// scale exponent 15 - e, e the exponent of the transform's largest |component|
+// (0 for an all-zero or NaN input)
+__device__ __forceinline__ int mfma_scale_exponent(const float* wmax) {
+    const float m4 = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    int ex = 0;
+    (void)frexpf(m4, &ex);  // m4 = f 2^ex, f in [0.5, 1)
+    return m4 > 0.f ? 15 - ex : 0;
+}
+
+// One 4096-point transform by a workgroup of 256 threads (4 waves, each
+// owning 4 groups of 16 butterflies per pass): x holds this lane's inputs
+// x[gg][mi] = in[i + 256 (4 h + mi)], i = 16 (4 wave + gg) + b, already
+// swapped for the backward direction; the outputs go to out (non-temporal).
+// Twiddles come from ltw, an LDS copy of the plan's pass twiddles.
 // FP16 range: the input is scaled by 2^(15 - e), e the exponent of its largest
 // |component|, so pass inputs stay below 2^15 and every split part is a normal
 // FP16 number; each pass can grow values by at most 16, and the FP16 twiddle
-// image holds W / 16, so the next pass's inputs are again below 2^15.  All
+// table holds W / 16, so the next pass's inputs are again below 2^15.  All
 // scalings are powers of two (exact) and are undone at the store.
-template <int P, int NS, int BWD>
-__global__ void __launch_bounds__(256) fft4096_mfma_kernel(KArgs a) {
+// The caller synchronises the workgroup before the call (LDS twiddles ready,
+// previous transform's LDS readers done).  MAXDONE: the caller has already
+// stored this transform's per-wave maxima in wmax (mfma_wave_absmax) before
+// that synchronisation.
+template <int P, int NS, int BWD, bool MAXDONE>
+__device__ __forceinline__ void mfma4096_transform(float2 (&x)[4][4], float2* lds, const float2* ltw, float* wmax,
+                                                   const typename MfOp<P>::V (&ar)[NS],
+                                                   const typename MfOp<P>::V (&ai)[NS], float2* out) {
     using G = Geo<12, 4, 256>;
-    using OP = MfOp<P>;
-    constexpr int N = 4096;
-    using TI = TwImage<G>;
-    __shared__ float2 lds[G::LDSN];
-    __shared__ float2 ltw[TI::total()];
-    __shared__ float wmax[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = lane & 15, h = lane >> 4;
-    typename OP::V ar[NS], ai[NS];
-    const float neg1 = a.batch < 0 ? 1.0f : -1.0f;  // -1 (batch >= 0), unknown to the compiler
-    dft16_operands<P, NS>(lane, ar, ai, neg1);
-    for (int e = tid; e < TI::total(); e += 256) ltw[e] = a.tw[e];
-
-    const long long f = blockIdx.x;
-    const float2* in = reinterpret_cast<const float2*>(a.in0) + f * N;
-    float2* out = reinterpret_cast<float2*>(a.out0) + f * N;
-    float2 x[4][4], y[4][4];
+    float2 y[4][4];
+    int sh = 0;
+    float s = 1.0f;  // 2^sh, applied inside the first pass's split
+    if constexpr (P == MF_F16) {
+        if (!MAXDONE) {
+            mfma_wave_absmax(x, wmax);
+            __syncthreads();  // per-wave maxima
+        }
+        sh = mfma_scale_exponent(wmax);
+        if (sh > 120) {  // tiny inputs: the part of the scale 2^sh a float cannot hold goes in first
 #pragma unroll
-    for (int gg = 0; gg < 4; ++gg) {
-        const int i = 16 * (wave * 4 + gg) + b;
+            for (int gg = 0; gg < 4; ++gg)
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi) {
-            const float2 t = ld_nt(in + i + 256 * (4 * h + mi));
-            x[gg][mi] = BWD ? swapri(t) : t;
+                for (int mi = 0; mi < 4; ++mi)
+                    x[gg][mi] = make_float2(ldexpf(x[gg][mi].x, sh - 120), ldexpf(x[gg][mi].y, sh - 120));
+            s = ldexpf(1.0f, 120);
+        } else {
+            s = ldexpf(1.0f, sh);
         }
     }
-    int sh = 0;
-    if constexpr (P == MF_F16) {
-        float mx = 0.f;
-#pragma unroll
-        for (int gg = 0; gg < 4; ++gg)
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi) mx = fmaxf(mx, fmaxf(fabsf(x[gg][mi].x), fabsf(x[gg][mi].y)));
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-        if (lane == 0) wmax[wave] = mx;
-    }
-    __syncthreads();  // twiddle image, per-wave maxima
-    if constexpr (P == MF_F16) {
-        const float mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        int ex = 0;
-        (void)frexpf(mx, &ex);  // mx = f 2^ex, f in [0.5, 1)
-        sh = mx > 0.f ? 15 - ex : 0;
-#pragma unroll
-        for (int gg = 0; gg < 4; ++gg)
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi) x[gg][mi] = make_float2(ldexpf(x[gg][mi].x, sh), ldexpf(x[gg][mi].y, sh));
-    }
-    // net power of two to undo at the store: the input scale and two 1/16 twiddle factors
+    // net power of two to undo at the store: the input scale and two 1/16
+    // twiddle factors; a float multiplier where 2^unscale is a normal float
     const int unscale = P == MF_F16 ? 8 - sh : 0;
+    const bool mul_unscale = unscale >= -126 && unscale <= 127;
+    const float us = mul_unscale ? ldexpf(1.0f, unscale) : 1.0f;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         if (p > 0) {
@@ -247,13 +297,18 @@ __global__ void __launch_bounds__(256) fft4096_mfma_kernel(KArgs a) {
                 for (int mi = 0; mi < 4; ++mi) {
                     const int m = 4 * h + mi;
                     const float2 t = lds[G::pad(i + 256 * m)];
-                    const float2 w = p == 1 ? img_twiddle<G, 1, P>(ltw, m, i & 15) : img_twiddle<G, 2, P>(ltw, m, i);
+                    const float2 w = p == 1 ? mf_twiddle<G, 1, P>(ltw, m, i & 15) : mf_twiddle<G, 2, P>(ltw, m, i);
                     x[gg][mi] = cmul(t, w);
                 }
             }
         }
+        if (P == MF_F16 && p == 0) {
 #pragma unroll
-        for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS>(x[gg], y[gg], ar, ai, neg1);
+            for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS, P == MF_F16>(x[gg], y[gg], ar, ai, s);
+        } else {
+#pragma unroll
+            for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS>(x[gg], y[gg], ar, ai);
+        }
         if (p < 2) {
             __syncthreads();  // all reads of the previous LDS contents are done
             const int lw = 4 * p;
@@ -272,10 +327,103 @@ __global__ void __launch_bounds__(256) fft4096_mfma_kernel(KArgs a) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     float2 o = y[gg][r];
-                    if constexpr (P == MF_F16) o = make_float2(ldexpf(o.x, unscale), ldexpf(o.y, unscale));
+                    if constexpr (P == MF_F16)
+                        o = mul_unscale ? make_float2(o.x * us, o.y * us)
+                                        : make_float2(ldexpf(o.x, unscale), ldexpf(o.y, unscale));
                     st_nt(out + i + 256 * (4 * h + r), BWD ? swapri(o) : o);
                 }
             }
+        }
+    }
+}
+
+// this lane's inputs of transform `in` (4096 complex, AoS)
+__device__ __forceinline__ void mfma4096_load(float2 (&x)[4][4], const float2* in) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = lane & 15, h = lane >> 4;
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+        const int i = 16 * (wave * 4 + gg) + b;
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) x[gg][mi] = ld_nt(in + i + 256 * (4 * h + mi));
+    }
+}
+
+// 4096-point CFFT, AoS in and out, one transform per workgroup.
+template <int P, int NS, int BWD>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFMA_STREAM_WAVES))) fft4096_mfma_kernel(KArgs a) {
+    using G = Geo<12, 4, 256>;
+    __shared__ float2 lds[G::LDSN];
+    __shared__ float2 ltw[G::tw_total()];
+    __shared__ float wmax[4];
+    const int tid = threadIdx.x;
+    typename MfOp<P>::V ar[NS], ai[NS];
+    dft16_operands<P, NS>(tid & 63, ar, ai);
+    for (int e = tid; e < G::tw_total(); e += 256) ltw[e] = a.tw[e];
+    const long long f = blockIdx.x;
+    float2 x[4][4];
+    mfma4096_load(x, reinterpret_cast<const float2*>(a.in0) + f * 4096);
+    if constexpr (BWD) {
+#pragma unroll
+        for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi) x[gg][mi] = swapri(x[gg][mi]);
+    }
+    __syncthreads();  // LDS twiddles
+    mfma4096_transform<P, NS, BWD, false>(x, lds, ltw, wmax, ar, ai, reinterpret_cast<float2*>(a.out0) + f * 4096);
+}
+
+// Persistent form: one workgroup per resident slot loops over transforms
+// f = blockIdx.x, + gridDim.x, ...; the next transform's input is loaded into
+// registers (non-temporal) while the current one computes, and the DFT
+// operands and the LDS twiddles are set up once per workgroup.  A lane with
+// no next transform prefetches from the sink instead, so the loop has no
+// branch around its loads.
+template <int P, int NS, int BWD>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFMA_STREAM_WAVES)))
+fft4096_mfma_stream_kernel(KArgs a) {
+    using G = Geo<12, 4, 256>;
+    __shared__ float2 lds[G::LDSN];
+    __shared__ float2 ltw[G::tw_total()];
+    __shared__ float wmax[4];
+    const int tid = threadIdx.x;
+    typename MfOp<P>::V ar[NS], ai[NS];
+    dft16_operands<P, NS>(tid & 63, ar, ai);
+    for (int e = tid; e < G::tw_total(); e += 256) ltw[e] = a.tw[e];
+    const float2* in = reinterpret_cast<const float2*>(a.in0);
+    const float2* sink = reinterpret_cast<const float2*>(a.sink);
+    long long f = blockIdx.x;
+    // two prefetch buffers used in turn (the loop body is unrolled twice), so
+    // the prefetched registers are never copied
+    float2 pa[4][4], pb[4][4];
+    mfma4096_load(pa, f < a.batch ? in + f * 4096 : sink);
+    // FP16: the per-wave maxima of a transform are taken from the prefetch
+    // registers at the end of the previous iteration, so the synchronisation
+    // at the top of the next one publishes them and the transform needs none
+    // of its own for the scale
+    if constexpr (P == MF_F16) mfma_wave_absmax(pa, wmax);
+    auto one = [&](long long f, float2 (&cur)[4][4], float2 (&nxt)[4][4]) {
+        if constexpr (BWD) {
+#pragma unroll
+            for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+                for (int mi = 0; mi < 4; ++mi) cur[gg][mi] = swapri(cur[gg][mi]);
+        }
+        const long long fn = f + gridDim.x;
+        mfma4096_load(nxt, fn < a.batch ? in + fn * 4096 : sink);
+        __syncthreads();  // LDS twiddles and maxima ready / previous transform's LDS readers done
+        mfma4096_transform<P, NS, BWD, true>(cur, lds, ltw, wmax, ar, ai,
+                                             reinterpret_cast<float2*>(a.out0) + f * 4096);
+        // every wave read wmax before the transform's first LDS exchange barrier
+        if constexpr (P == MF_F16) mfma_wave_absmax(nxt, wmax);
+    };
+    if (f < a.batch) {
+        one(f, pa, pb);
+        for (f += gridDim.x; f < a.batch; f += gridDim.x) {
+            one(f, pb, pa);
+            f += gridDim.x;
+            if (f >= a.batch) break;
+            one(f, pa, pb);
         }
     }
 }

@@ -127,8 +127,9 @@ MacroEntry make_macro_entry() {
 
 }  // namespace zl
 
-// MFMA split-precision 4096-point kernel for a precision code, or null
-void* zl_mfma_kernel(int prec, int backward);
+// MFMA split-precision 4096-point kernel for a precision code (one transform
+// per workgroup, or the persistent streaming form), or null
+void* zl_mfma_kernel(int prec, int backward, int streaming);
 
 // ragged-batch kernels (zlfft_ragged_k.hip)
 void* zl_ragged_kernel();
