@@ -149,13 +149,11 @@ __global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi
     out[idx] = make_float2((float)cs, (float)sn);
 }
 
-// FP16 MFMA kernels' twiddle table: entries [0, lo_n) divided by 16 (exact),
-// entries [lo_n, lo_n + hi_n) copied.
-__global__ void scale_twiddles16(const float2* in, float2* out, int lo_n, int hi_n) {
+// FP16 MFMA kernels' twiddle table: every pass twiddle divided by 16 (exact).
+__global__ void scale_twiddles16(const float2* in, float2* out, int count) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= lo_n + hi_n) return;
-    const float s = idx < lo_n ? 0.0625f : 1.0f;
-    out[idx] = make_float2(in[idx].x * s, in[idx].y * s);
+    if (idx >= count) return;
+    out[idx] = make_float2(in[idx].x * 0.0625f, in[idx].y * 0.0625f);
 }
 
 // Macro twiddle table of an HBM pass: out[idx] = W_{2^log_lw}^{idx << shift},
@@ -562,7 +560,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         if (precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT) {
             e = hipMalloc(&p->d_tw16, sizeof(float2) * (size_t)ent.tw_total);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
-            scale_twiddles16<<<(ent.tw_total + 255) / 256, 256, 0, p->stream>>>(p->d_tw, p->d_tw16, ent.tw_total, 0);
+            scale_twiddles16<<<(ent.tw_total + 255) / 256, 256, 0, p->stream>>>(p->d_tw, p->d_tw16, ent.tw_total);
         }
         e = hipGetLastError();
         if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));

@@ -215,6 +215,7 @@ __device__ __forceinline__ float2 mf_twiddle(const float2* ltw, int m, int k) {
     const float2 t = ltw[G::tw_off(P) + (m > 0 ? m - 1 : 0) * w + k];
     return m > 0 ? t : make_float2(PREC == MF_F16 ? 0.0625f : 1.f, 0.f);
 }
+constexpr int mf_tw_total() { return Geo<12, 4, 256>::tw_total(); }
 
 // largest |component| of this wave's inputs into wmax[wave]: bit patterns
 // with the sign cleared compare as unsigned integers in the order of the
@@ -354,12 +355,12 @@ template <int P, int NS, int BWD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFMA_STREAM_WAVES))) fft4096_mfma_kernel(KArgs a) {
     using G = Geo<12, 4, 256>;
     __shared__ float2 lds[G::LDSN];
-    __shared__ float2 ltw[G::tw_total()];
+    __shared__ float2 ltw[mf_tw_total()];
     __shared__ float wmax[4];
     const int tid = threadIdx.x;
     typename MfOp<P>::V ar[NS], ai[NS];
     dft16_operands<P, NS>(tid & 63, ar, ai);
-    for (int e = tid; e < G::tw_total(); e += 256) ltw[e] = a.tw[e];
+    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = a.tw[e];
     const long long f = blockIdx.x;
     float2 x[4][4];
     mfma4096_load(x, reinterpret_cast<const float2*>(a.in0) + f * 4096);
@@ -384,12 +385,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFM
 fft4096_mfma_stream_kernel(KArgs a) {
     using G = Geo<12, 4, 256>;
     __shared__ float2 lds[G::LDSN];
-    __shared__ float2 ltw[G::tw_total()];
+    __shared__ float2 ltw[mf_tw_total()];
     __shared__ float wmax[4];
     const int tid = threadIdx.x;
     typename MfOp<P>::V ar[NS], ai[NS];
     dft16_operands<P, NS>(tid & 63, ar, ai);
-    for (int e = tid; e < G::tw_total(); e += 256) ltw[e] = a.tw[e];
+    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = a.tw[e];
     const float2* in = reinterpret_cast<const float2*>(a.in0);
     const float2* sink = reinterpret_cast<const float2*>(a.sink);
     long long f = blockIdx.x;
