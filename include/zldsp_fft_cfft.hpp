@@ -6,7 +6,8 @@
 // (std::array<F*, 2>) buffers.  Calls are synchronous and accept host or
 // device pointers; the work runs on the GPU through libzlfft (zlfft.h).
 // Differences: invalid orders and GPU failures throw std::runtime_error (the
-// reference has undefined behaviour there); only F = float is available.
+// reference has undefined behaviour there).  F is float or double, as in the
+// reference; other types fail to compile.
 #pragma once
 
 #include <array>
@@ -30,7 +31,7 @@ inline void check(int rc) {
 // their buffers are AlignedFreeUniquePtr members).
 class Plan {
 public:
-    Plan(int kind, size_t order) { check(zlfft_plan_create(&h_, kind, ZLFFT_PREC_F32, order)); }
+    Plan(int kind, int precision, size_t order) { check(zlfft_plan_create(&h_, kind, precision, order)); }
     ~Plan() {
         if (h_) zlfft_plan_destroy(h_);
     }
@@ -50,15 +51,21 @@ private:
     zlfft_plan h_ = nullptr;
 };
 
+// plan precision of the element type F
+template <typename F>
+constexpr int precision_of() {
+    static_assert(std::is_same_v<F, float> || std::is_same_v<F, double>, "zldsp::fft supports float and double");
+    return std::is_same_v<F, double> ? ZLFFT_PREC_F64 : ZLFFT_PREC_F32;
+}
+
 }  // namespace detail
 
 template <typename F>
 class CFFT {
-    static_assert(std::is_same_v<F, float>, "this build provides CFFT<float>");
     using C = std::complex<F>;
 
 public:
-    explicit CFFT(const size_t cfft_order) : plan_(ZLFFT_KIND_CFFT, cfft_order) {}
+    explicit CFFT(const size_t cfft_order) : plan_(ZLFFT_KIND_CFFT, detail::precision_of<F>(), cfft_order) {}
 
     [[nodiscard]] size_t get_size() const { return zlfft_plan_get_size(plan_.get()); }
 

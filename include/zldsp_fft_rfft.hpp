@@ -4,7 +4,7 @@
 // an explicit order constructor (N = 2^order reals), get_size() = N,
 // get_order(), forward (real -> N/2+1 bins, AoS or SoA), backward
 // (N/2+1 bins -> N reals, scaled by N/2 like the reference, the imaginary
-// parts of bins 0 and N/2 ignored) and forward_sqr_mag (N/2+1 floats).
+// parts of bins 0 and N/2 ignored) and forward_sqr_mag (N/2+1 values of F).
 // Calls are synchronous and accept host or device pointers.
 #pragma once
 
@@ -14,11 +14,10 @@ namespace zldsp::fft {
 
 template <typename F>
 class RFFT {
-    static_assert(std::is_same_v<F, float>, "this build provides RFFT<float>");
     using C = std::complex<F>;
 
 public:
-    explicit RFFT(const size_t rfft_order) : plan_(ZLFFT_KIND_RFFT, rfft_order) {}
+    explicit RFFT(const size_t rfft_order) : plan_(ZLFFT_KIND_RFFT, detail::precision_of<F>(), rfft_order) {}
 
     [[nodiscard]] size_t get_size() const { return zlfft_plan_get_size(plan_.get()); }
 

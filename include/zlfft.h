@@ -60,8 +60,12 @@ extern "C" {
 #define ZLFFT_KIND_CFFT 0
 #define ZLFFT_KIND_RFFT 1
 
-/* arithmetic of a plan.  ZLFFT_PREC_F32 is the plain path, for every kind,
- * order and layout.  The MFMA precisions run each radix-16 pass of the
+/* arithmetic of a plan.  ZLFFT_PREC_F32 is the plain path (CFFT<float> /
+ * RFFT<float>), for every kind, order and layout; ZLFFT_PREC_F64 is the same
+ * over double (CFFT<double> / RFFT<double>, reference src/zldsp_fft_cfft.hpp:11,
+ * src/zldsp_fft_rfft.hpp:11: the classes are templates on F), CFFT orders up to
+ * ZLFFT_MAX_CFFT_ORDER_F64; its buffers hold double (AoS: std::complex<double>).
+ * The MFMA precisions run each radix-16 pass of the
  * 4096-point CFFT (order 12, AoS in and out) as a 16x16 DFT-matrix product on
  * the matrix cores with FP32 accumulate; the operands are FP16 or BF16, whole
  * or split into 2 (hi + lo) or 3 (hi + mid + lo) parts.  Other orders, kinds or
@@ -72,6 +76,7 @@ extern "C" {
 #define ZLFFT_PREC_BF16_MFMA 3
 #define ZLFFT_PREC_BF16_MFMA_SPLIT 4
 #define ZLFFT_PREC_BF16_MFMA_SPLIT3 5
+#define ZLFFT_PREC_F64 6
 
 /* user layouts */
 #define ZLFFT_AOS 0
@@ -83,13 +88,15 @@ extern "C" {
 
 /* largest supported order: CFFT 2^ZLFFT_MAX_CFFT_ORDER points (RFFT one more) */
 #define ZLFFT_MAX_CFFT_ORDER 28
+#define ZLFFT_MAX_CFFT_ORDER_F64 27
 
 typedef struct zlfft_plan_s *zlfft_plan;
 
 /* Replaces the constructors CFFT<F>::CFFT(size_t order)
  * (reference src/zldsp_fft_cfft.hpp:19-23) and RFFT<F>::RFFT(size_t order)
  * (reference src/zldsp_fft_rfft.hpp:24-53).  Builds the device plan: kernel
- * choice and FP32 twiddle tables generated on the GPU in FP64.
+ * choice and twiddle tables generated on the GPU in FP64 (rounded to FP32 for
+ * the FP32 and MFMA precisions).
  * kind: ZLFFT_KIND_CFFT (size 2^order complex) or ZLFFT_KIND_RFFT
  * (size 2^order real, order >= 1). */
 int zlfft_plan_create(zlfft_plan *plan, int kind, int precision, size_t order);
@@ -108,15 +115,18 @@ size_t zlfft_plan_get_order(zlfft_plan plan);
 int zlfft_cfft_exec(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
                     int out_layout, void *out0, void *out1, size_t batch, void *stream);
 
+/* Real-side buffers (`in` of forward, `out` of backward and sqr-mag) hold F
+ * values: float for ZLFFT_PREC_F32 plans, double for ZLFFT_PREC_F64 plans. */
+
 /* Batched device-pointer RFFT<F>::forward (reference src/zldsp_fft_rfft.hpp:68-79). */
-int zlfft_rfft_forward_exec(zlfft_plan plan, const float *in, int out_layout, void *out0, void *out1,
+int zlfft_rfft_forward_exec(zlfft_plan plan, const void *in, int out_layout, void *out0, void *out1,
                             size_t batch, void *stream);
 
 /* Batched device-pointer RFFT<F>::forward_sqr_mag (reference src/zldsp_fft_rfft.hpp:104-109). */
-int zlfft_rfft_forward_sqr_mag_exec(zlfft_plan plan, const float *in, float *out, size_t batch, void *stream);
+int zlfft_rfft_forward_sqr_mag_exec(zlfft_plan plan, const void *in, void *out, size_t batch, void *stream);
 
 /* Batched device-pointer RFFT<F>::backward (reference src/zldsp_fft_rfft.hpp:86-97). */
-int zlfft_rfft_backward_exec(zlfft_plan plan, int in_layout, const void *in0, const void *in1, float *out,
+int zlfft_rfft_backward_exec(zlfft_plan plan, int in_layout, const void *in0, const void *in1, void *out,
                              size_t batch, void *stream);
 
 /* Synchronous single-transform forms of the four calls above for host or
@@ -124,9 +134,9 @@ int zlfft_rfft_backward_exec(zlfft_plan plan, int in_layout, const void *in0, co
  * the result in out). */
 int zlfft_cfft_execute(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
                        int out_layout, void *out0, void *out1);
-int zlfft_rfft_forward(zlfft_plan plan, const float *in, int out_layout, void *out0, void *out1);
-int zlfft_rfft_forward_sqr_mag(zlfft_plan plan, const float *in, float *out);
-int zlfft_rfft_backward(zlfft_plan plan, int in_layout, const void *in0, const void *in1, float *out);
+int zlfft_rfft_forward(zlfft_plan plan, const void *in, int out_layout, void *out0, void *out1);
+int zlfft_rfft_forward_sqr_mag(zlfft_plan plan, const void *in, void *out);
+int zlfft_rfft_backward(zlfft_plan plan, int in_layout, const void *in0, const void *in1, void *out);
 
 /* ---- ragged batches ----
  * Many transforms of different orders, kinds, directions and layouts run as
@@ -163,7 +173,8 @@ int zlfft_ragged_destroy(zlfft_ragged ragged);
 
 /* Copies the plan's RFFT post/pre twiddles (wc, ws) for k = 1..N/4 as 2*(N/4)
  * floats to host memory (reference src/common/zldsp_fft_common_init.hpp:347-388);
- * for tests that compare them with the FP64 oracle. */
+ * for tests that compare them with the FP64 oracle.  FP32 plans only
+ * (ZLFFT_ERR_UNSUPPORTED for FP64 plans). */
 int zlfft_plan_copy_rfft_twiddles(zlfft_plan plan, float *host_out, size_t n_floats);
 
 /* Description of the last error on the calling thread ("" if none). */

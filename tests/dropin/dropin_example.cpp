@@ -91,6 +91,29 @@ int main() {
     const double e4 = std::sqrt(e4n / e4d), e5 = std::sqrt(e5n / e5d);
     std::printf("cfft fwd %.3e  round trip %.3e  rfft fwd %.3e  rfft round trip %.3e  sqr_mag %.3e  exact-check failures %d\n",
                 e1, e2, e3, e4, e5, bad);
+    // the same API over double (CFFT<double> / RFFT<double>)
+    zldsp::fft::CFFT<double> cfft64(order);
+    std::vector<std::complex<double>> in64(x), out64(n), back64(n);
+    cfft64.forward(in64.data(), out64.data());
+    const double e6 = rel(out64, X);
+    cfft64.backward(out64.data(), back64.data());
+    const double e7 = rel(back64, nx);
+    zldsp::fft::RFFT<double> rfft64(11);
+    std::vector<double> real64(real.begin(), real.end()), back_real64(nr), mag64(nr / 2 + 1);
+    std::vector<std::complex<double>> bins64(nr / 2 + 1);
+    rfft64.forward(real64.data(), bins64.data());
+    rfft64.forward_sqr_mag(real64.data(), mag64.data());
+    rfft64.backward(bins64.data(), back_real64.data());
+    const double e8 = rel(bins64, xb);
+    double e9n = 0, e9d = 0;
+    for (size_t i = 0; i < nr; ++i) {
+        e9n += std::pow(back_real64[i] - 0.5 * nr * real64[i], 2);
+        e9d += std::pow(0.5 * nr * real64[i], 2);
+    }
+    const double e9 = std::sqrt(e9n / e9d);
+    bad += (bins64[0].imag() != 0.0) + (bins64[nr / 2].imag() != 0.0);
+    for (size_t k = 0; k <= nr / 2; ++k) bad += std::abs(mag64[k] - std::norm(bins64[k])) > 1e-9 * (1 + std::norm(bins64[k]));
+    std::printf("double: cfft fwd %.3e  round trip %.3e  rfft fwd %.3e  rfft round trip %.3e\n", e6, e7, e8, e9);
     // invalid use is reported, not undefined
     bool threw = false;
     try {
@@ -98,7 +121,8 @@ int main() {
     } catch (const std::runtime_error&) {
         threw = true;
     }
-    const bool ok = e1 < 1e-6 && e2 < 1e-6 && e3 < 1e-6 && e4 < 1e-6 && e5 < 2e-6 && bad == 0 && threw;
+    const bool ok = e1 < 1e-6 && e2 < 1e-6 && e3 < 1e-6 && e4 < 1e-6 && e5 < 2e-6 && e6 < 1e-12 && e7 < 1e-12 &&
+                    e8 < 1e-12 && e9 < 1e-12 && bad == 0 && threw;
     std::printf(ok ? "DROPIN OK\n" : "DROPIN FAILED\n");
     return ok ? 0 : 1;
 }

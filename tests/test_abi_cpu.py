@@ -38,6 +38,9 @@ def test_constants_match_header():
     assert int(consts["ZLFFT_FORWARD"]) == zf.FORWARD and int(consts["ZLFFT_BACKWARD"]) == zf.BACKWARD
     assert int(consts["ZLFFT_KIND_CFFT"]) == zf.KIND_CFFT and int(consts["ZLFFT_KIND_RFFT"]) == zf.KIND_RFFT
     assert int(consts["ZLFFT_MAX_CFFT_ORDER"]) == zf.MAX_CFFT_ORDER
+    assert int(consts["ZLFFT_MAX_CFFT_ORDER_F64"]) == zf.MAX_CFFT_ORDER_F64
+    assert int(consts["ZLFFT_PREC_F64"]) == zf.PREC_F64
+    assert int(consts["ZLFFT_PREC_BF16_MFMA_SPLIT3"]) == zf.PREC_BF16_MFMA_SPLIT3
 
 
 def _create(kind, prec, order):
@@ -53,6 +56,9 @@ def _create(kind, prec, order):
     (zf.KIND_RFFT, 0, 30, 1),
     (7, 0, 10, 5),                     # unknown kind
     (zf.KIND_CFFT, 9, 10, 6),          # unknown precision
+    (zf.KIND_CFFT, 6, 28, 1),          # FP64 above its maximum order (27)
+    (zf.KIND_RFFT, 6, 29, 1),
+    (zf.KIND_RFFT, 2, 12, 6),          # MFMA precisions are CFFT-only
 ])
 def test_plan_create_rejects_invalid_arguments(kind, prec, order, code):
     rc, h, msg = _create(kind, prec, order)

@@ -1,6 +1,7 @@
 """The drop-in C++ classes (include/zldsp_fft_cfft.hpp, zldsp_fft_rfft.hpp):
 the README usage of the reference compiles against them, and, on the GPU, the
-README-style program's results match a double-precision DFT."""
+README-style program's results match a double-precision DFT, for CFFT<float>,
+RFFT<float>, CFFT<double> and RFFT<double>."""
 import os
 import subprocess
 
@@ -17,11 +18,18 @@ def test_dropin_headers_compile():
     assert r.returncode == 0, r.stderr
 
 
-def test_dropin_header_rejects_double():
-    code = '#include "zldsp_fft_cfft.hpp"\nint main() { zldsp::fft::CFFT<double> c(4); return 0; }\n'
-    r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-x", "c++", "-I", os.path.join(ROOT, "include"), "-"],
-                       input=code, capture_output=True, text=True)
-    assert r.returncode != 0 and "CFFT<float>" in r.stderr
+def test_dropin_header_rejects_other_types():
+    """float and double compile (the reference's two instantiations); other
+    element types fail at compile time with a message."""
+    inc = os.path.join(ROOT, "include")
+    for t, ok in (("float", True), ("double", True), ("long double", False), ("int", False)):
+        code = ('#include "zldsp_fft_rfft.hpp"\nint main() { zldsp::fft::CFFT<%s> c(4); '
+                'zldsp::fft::RFFT<%s> r(4); return 0; }\n' % (t, t))
+        r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-x", "c++", "-I", inc, "-"],
+                           input=code, capture_output=True, text=True)
+        assert (r.returncode == 0) == ok, (t, r.stderr[-400:])
+        if not ok:
+            assert "supports float and double" in r.stderr
 
 
 @pytest.mark.gpu

@@ -40,9 +40,11 @@ KIND_CFFT, KIND_RFFT = 0, 1
 PREC_F32 = 0
 PREC_F16_MFMA, PREC_F16_MFMA_SPLIT = 1, 2
 PREC_BF16_MFMA, PREC_BF16_MFMA_SPLIT, PREC_BF16_MFMA_SPLIT3 = 3, 4, 5
+PREC_F64 = 6
 AOS, SOA = 0, 1
 FORWARD, BACKWARD = 0, 1
 MAX_CFFT_ORDER = 28
+MAX_CFFT_ORDER_F64 = 27
 MAX_LDS_ORDER = 14
 
 _ERRNAMES = {1: "INVALID_ORDER", 2: "NULL", 3: "ALLOC", 4: "HIP", 5: "INVALID_ARG", 6: "UNSUPPORTED"}
@@ -155,48 +157,48 @@ def _is_soa(buf) -> bool:
     return isinstance(buf, (tuple, list))
 
 
-def _complex_view(buf, name: str):
-    """AoS buffer: complex64 or float32 with interleaved re, im."""
+def _dtype_name(buf) -> str:
+    return str(buf.dtype).replace("torch.", "")
+
+
+def _complex_view(buf, name: str, f64: bool = False):
+    """AoS buffer: complex64 or float32 with interleaved re, im (FP64 plans:
+    complex128 or interleaved float64)."""
     _contig(buf, name)
-    if _is_torch(buf):
-        import torch
-        if buf.dtype not in (torch.complex64, torch.float32):
-            raise TypeError(f"{name}: AoS buffers are complex64 (or interleaved float32)")
-    elif buf.dtype not in (np.complex64, np.float32):
-        raise TypeError(f"{name}: AoS buffers are complex64 (or interleaved float32)")
+    ok = ("complex128", "float64") if f64 else ("complex64", "float32")
+    if _dtype_name(buf) not in ok:
+        raise TypeError(f"{name}: AoS buffers are {ok[0]} (or interleaved {ok[1]})")
     return buf
 
 
-def _float_view(buf, name: str):
+def _float_view(buf, name: str, f64: bool = False):
     _contig(buf, name)
-    if _is_torch(buf):
-        import torch
-        if buf.dtype != torch.float32:
-            raise TypeError(f"{name}: float32 required")
-    elif buf.dtype != np.float32:
-        raise TypeError(f"{name}: float32 required")
+    want = "float64" if f64 else "float32"
+    if _dtype_name(buf) != want:
+        raise TypeError(f"{name}: {want} required")
     return buf
 
 
 class _Side:
     """One side (input or output) of a call: layout, pointers, element count."""
 
-    def __init__(self, buf, name: str):
+    def __init__(self, buf, name: str, f64: bool = False):
+        ss = 8 if f64 else 4
         if _is_soa(buf):
             if len(buf) != 2:
                 raise ValueError(f"{name}: SoA buffers are (real, imag)")
-            re, im = _float_view(buf[0], name + "[0]"), _float_view(buf[1], name + "[1]")
+            re, im = _float_view(buf[0], name + "[0]", f64), _float_view(buf[1], name + "[1]", f64)
             if _nbytes(re) != _nbytes(im):
                 raise ValueError(f"{name}: real and imag sizes differ")
             self.layout, self.p0, self.p1 = SOA, _ptr(re), _ptr(im)
-            self.ncomplex = _nbytes(re) // 4
+            self.ncomplex = _nbytes(re) // ss
             self.dev = _is_dev(re) and _is_dev(im)
             self.host = not (_is_dev(re) or _is_dev(im))
             self.ref = (re, im)
         else:
-            b = _complex_view(buf, name)
+            b = _complex_view(buf, name, f64)
             self.layout, self.p0, self.p1 = AOS, _ptr(b), None
-            self.ncomplex = _nbytes(b) // 8
+            self.ncomplex = _nbytes(b) // (2 * ss)
             self.dev = _is_dev(b)
             self.host = not self.dev
             self.ref = (b,)
@@ -224,6 +226,9 @@ class _Plan:
         h = ctypes.c_void_p()
         _check(self._lib.zlfft_plan_create(ctypes.byref(h), kind, int(precision), int(order)))
         self._h = h
+        self.precision = int(precision)
+        self._f64 = self.precision == PREC_F64
+        self._ss = 8 if self._f64 else 4
 
     def close(self) -> None:
         if getattr(self, "_h", None) and self._h.value:
@@ -256,15 +261,16 @@ class _Plan:
 class CFFT(_Plan):
     """``zldsp::fft::CFFT<float>`` (reference ``src/zldsp_fft_cfft.hpp:11-118``).
 
-    ``precision`` selects the arithmetic: PREC_F32 (the plain path) or one of the
-    MFMA split precisions (4096-point AoS transforms only)."""
+    ``precision`` selects the arithmetic: PREC_F32 (the plain path), PREC_F64
+    (``CFFT<double>``: complex128 / float64 buffers) or one of the MFMA split
+    precisions (4096-point AoS transforms only)."""
 
     def __init__(self, order: int, precision: int = PREC_F32):
         super().__init__(KIND_CFFT, order, precision)
 
     def _run(self, direction: int, in_buffer, out_buffer) -> None:
         n = self.get_size()
-        si, so = _Side(in_buffer, "in_buffer"), _Side(out_buffer, "out_buffer")
+        si, so = _Side(in_buffer, "in_buffer", self._f64), _Side(out_buffer, "out_buffer", self._f64)
         if si.dev and so.dev:
             b = _batch(si.ncomplex, n, "in_buffer")
             if _batch(so.ncomplex, n, "out_buffer") != b:
@@ -286,17 +292,18 @@ class CFFT(_Plan):
 
 
 class RFFT(_Plan):
-    """``zldsp::fft::RFFT<float>`` (reference ``src/zldsp_fft_rfft.hpp:11-139``)."""
+    """``zldsp::fft::RFFT<float>`` (reference ``src/zldsp_fft_rfft.hpp:11-139``);
+    ``precision=PREC_F64`` gives ``RFFT<double>`` (float64 / complex128 buffers)."""
 
-    def __init__(self, order: int):
-        super().__init__(KIND_RFFT, order)
+    def __init__(self, order: int, precision: int = PREC_F32):
+        super().__init__(KIND_RFFT, order, precision)
 
     def forward(self, in_buffer, out_buffer) -> None:
         """N reals -> N/2+1 bins, AoS or SoA (reference ``src/zldsp_fft_rfft.hpp:68-79``)."""
         n = self.get_size()
         nb = n // 2 + 1
-        x = _float_view(in_buffer, "in_buffer")
-        so = _Side(out_buffer, "out_buffer")
+        x = _float_view(in_buffer, "in_buffer", self._f64)
+        so = _Side(out_buffer, "out_buffer", self._f64)
         if _is_dev(x) and so.dev:
             b = _batch(x.numel(), n, "in_buffer")
             if _batch(so.ncomplex, nb, "out_buffer") != b:
@@ -304,7 +311,7 @@ class RFFT(_Plan):
             _check(self._lib.zlfft_rfft_forward_exec(self._h, _ptr(x), so.layout, so.p0, so.p1, b,
                                                      _stream_of(x, *so.ref)))
             return
-        if _nbytes(x) != 4 * n or so.ncomplex != nb:
+        if _nbytes(x) != self._ss * n or so.ncomplex != nb:
             raise ValueError(f"host buffers must hold {n} reals and {nb} bins")
         _check(self._lib.zlfft_rfft_forward(self._h, _ptr(x), so.layout, so.p0, so.p1))
 
@@ -312,8 +319,8 @@ class RFFT(_Plan):
         """N/2+1 bins (AoS or SoA) -> N reals (reference ``src/zldsp_fft_rfft.hpp:86-97``)."""
         n = self.get_size()
         nb = n // 2 + 1
-        si = _Side(in_buffer, "in_buffer")
-        y = _float_view(out_buffer, "out_buffer")
+        si = _Side(in_buffer, "in_buffer", self._f64)
+        y = _float_view(out_buffer, "out_buffer", self._f64)
         if si.dev and _is_dev(y):
             b = _batch(si.ncomplex, nb, "in_buffer")
             if _batch(y.numel(), n, "out_buffer") != b:
@@ -321,7 +328,7 @@ class RFFT(_Plan):
             _check(self._lib.zlfft_rfft_backward_exec(self._h, si.layout, si.p0, si.p1, _ptr(y), b,
                                                       _stream_of(*si.ref, y)))
             return
-        if si.ncomplex != nb or _nbytes(y) != 4 * n:
+        if si.ncomplex != nb or _nbytes(y) != self._ss * n:
             raise ValueError(f"host buffers must hold {nb} bins and {n} reals")
         _check(self._lib.zlfft_rfft_backward(self._h, si.layout, si.p0, si.p1, _ptr(y)))
 
@@ -329,15 +336,15 @@ class RFFT(_Plan):
         """N reals -> N/2+1 squared magnitudes (reference ``src/zldsp_fft_rfft.hpp:104-109``)."""
         n = self.get_size()
         nb = n // 2 + 1
-        x = _float_view(in_buffer, "in_buffer")
-        y = _float_view(out_buffer, "out_buffer")
+        x = _float_view(in_buffer, "in_buffer", self._f64)
+        y = _float_view(out_buffer, "out_buffer", self._f64)
         if _is_dev(x) and _is_dev(y):
             b = _batch(x.numel(), n, "in_buffer")
             if _batch(y.numel(), nb, "out_buffer") != b:
                 raise ValueError("in_buffer and out_buffer hold different batch sizes")
             _check(self._lib.zlfft_rfft_forward_sqr_mag_exec(self._h, _ptr(x), _ptr(y), b, _stream_of(x, y)))
             return
-        if _nbytes(x) != 4 * n or _nbytes(y) != 4 * nb:
+        if _nbytes(x) != self._ss * n or _nbytes(y) != self._ss * nb:
             raise ValueError(f"host buffers must hold {n} reals and {nb} floats")
         _check(self._lib.zlfft_rfft_forward_sqr_mag(self._h, _ptr(x), _ptr(y)))
 

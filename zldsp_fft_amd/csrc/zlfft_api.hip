@@ -88,6 +88,8 @@ int hip_fail(hipError_t e, const char* what) {
 
 zl::OrderTable g_tab[zl::kMaxLdsOrder + 1];
 zl::MacroTable g_macro[zl::kMaxMacroLogL + 1];
+zl::OrderTable g_tab64[zl::kMaxLdsOrder64 + 1];  // FP64
+zl::MacroTable g_macro64[zl::kMaxMacroLogL + 1];
 std::once_flag g_tab_once;
 
 void fill_tables() {
@@ -111,12 +113,33 @@ void fill_tables() {
     zl_fill_macro_8(g_macro[8]);
     zl_fill_macro_9(g_macro[9]);
     zl_fill_macro_10(g_macro[10]);
+    zl_fill_order64_0(g_tab64[0]);
+    zl_fill_order64_1(g_tab64[1]);
+    zl_fill_order64_2(g_tab64[2]);
+    zl_fill_order64_3(g_tab64[3]);
+    zl_fill_order64_4(g_tab64[4]);
+    zl_fill_order64_5(g_tab64[5]);
+    zl_fill_order64_6(g_tab64[6]);
+    zl_fill_order64_7(g_tab64[7]);
+    zl_fill_order64_8(g_tab64[8]);
+    zl_fill_order64_9(g_tab64[9]);
+    zl_fill_order64_10(g_tab64[10]);
+    zl_fill_order64_11(g_tab64[11]);
+    zl_fill_order64_12(g_tab64[12]);
+    zl_fill_order64_13(g_tab64[13]);
+    zl_fill_macro64_6(g_macro64[6]);
+    zl_fill_macro64_7(g_macro64[7]);
+    zl_fill_macro64_8(g_macro64[8]);
+    zl_fill_macro64_9(g_macro64[9]);
+    zl_fill_macro64_10(g_macro64[10]);
 }
 
 // Stockham twiddles of one pass: out[(m - 1) w + k] = W_{r w}^{m k}, computed
 // in FP64 (exact dyadic angle, sincospi) and rounded to FP32, as the reference
 // computes its tables in double and casts (src/common/zldsp_fft_common_init.hpp:84-90).
-__global__ void gen_pass_twiddles(float2* out, int lr, int lw, int count) {
+template <class CX>
+__global__ void gen_pass_twiddles(CX* out, int lr, int lw, int count) {
+    using S = typename zl::CxT<CX>::S;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= count) return;
     const int w = 1 << lw;
@@ -125,7 +148,7 @@ __global__ void gen_pass_twiddles(float2* out, int lr, int lw, int count) {
     const long long e = (m * k) & (rw - 1);
     double s, c;
     sincospi(-2.0 * (double)e / (double)rw, &s, &c);
-    out[idx] = make_float2((float)c, (float)s);
+    out[idx] = zl::CxT<CX>::mk((S)c, (S)s);
 }
 
 // LDS twiddle image of one Stockham pass (radix 2^lr, width 2^lw): the
@@ -158,25 +181,29 @@ __global__ void scale_twiddles16(const float2* in, float2* out, int count) {
 
 // Macro twiddle table of an HBM pass: out[idx] = W_{2^log_lw}^{idx << shift},
 // FP64 on the GPU, rounded to FP32.
-__global__ void gen_macro_twiddles(float2* out, int log_lw, int shift, int count) {
+template <class CX>
+__global__ void gen_macro_twiddles(CX* out, int log_lw, int shift, int count) {
+    using S = typename zl::CxT<CX>::S;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= count) return;
     const long long mod = 1ll << log_lw;
     const long long e = ((long long)idx << shift) & (mod - 1);
     double s, c;
     sincospi(-2.0 * (double)e / (double)mod, &s, &c);
-    out[idx] = make_float2((float)c, (float)s);
+    out[idx] = zl::CxT<CX>::mk((S)c, (S)s);
 }
 
 // RFFT pre/post twiddles, reference src/common/zldsp_fft_common_init.hpp:374-386:
 // a = k * (2 / N); wc = (F)(0.5 cospi(a)), ws = (F)(0.5 sinpi(a) + 0.5), k = 1..N/4.
-__global__ void gen_rfft_twiddles(float2* out, long long nreal, int count) {
+template <class CX>
+__global__ void gen_rfft_twiddles(CX* out, long long nreal, int count) {
+    using S = typename zl::CxT<CX>::S;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= count) return;
     const double a = (double)(idx + 1) * (2.0 / (double)nreal);
     double s, c;
     sincospi(a, &s, &c);
-    out[idx] = make_float2((float)(0.5 * c), (float)(0.5 * s + 0.5));
+    out[idx] = zl::CxT<CX>::mk((S)(0.5 * c), (S)(0.5 * s + 0.5));
 }
 
 }  // namespace
@@ -187,23 +214,25 @@ struct zlfft_plan_s {
     int order = 0;   // user order
     int corder = 0;  // inner CFFT order
     int device = 0;
-    const zl::OrderTable* tab = nullptr;  // LDS kernels (corder <= kMaxLdsOrder)
-    float2* d_tw = nullptr;
+    bool f64 = false;           // FP64 plan (CFFT<double> / RFFT<double>)
+    size_t ss = 4;              // bytes per real scalar (4 or 8)
+    const zl::OrderTable* tab = nullptr;  // LDS kernels (corder <= kMaxLdsOrder[64])
+    void* d_tw = nullptr;       // pass twiddles, complex type of the plan
     float2* d_img = nullptr;    // LDS twiddle image for the streaming kernels
     float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
     float2* d_tw16 = nullptr;   // pass twiddles / 16 for the FP16 MFMA kernels
     int cus = 0;                // compute units of the device
     int stream_blocks[4][2][2] = {};  // resident workgroups per CU of each streaming kernel
     int mfma_blocks = 0;              // resident workgroups per CU of the persistent MFMA kernel
-    float2* d_rtw = nullptr;
+    void* d_rtw = nullptr;      // RFFT (wc, ws), complex type of the plan
     int n_rtw = 0;
     // multi-pass transforms (corder > kMaxLdsOrder)
     bool macro = false;
     int mp = 0;                 // HBM passes
     int logl[4] = {}, logw[4] = {}, logh[4] = {};
-    float2* d_itw[4] = {};      // inner Stockham twiddles per pass
-    float2* d_twa[4] = {};      // macro twiddle tables per pass (p >= 1)
-    float2* d_twb[4] = {};
+    void* d_itw[4] = {};        // inner Stockham twiddles per pass
+    void* d_twa[4] = {};        // macro twiddle tables per pass (p >= 1)
+    void* d_twb[4] = {};
     void* d_work = nullptr;     // intermediate buffers, grown on demand
     size_t work_bytes = 0;
     // synchronous calls on host pointers: staging buffers and a stream
@@ -238,15 +267,15 @@ int ensure_work(zlfft_plan p, size_t bytes) {
 // Multi-pass CFFT of order p->corder on `batch` transforms: pass 0 reads the
 // user layout lin, the last pass writes lout, intermediates ping-pong in work.
 int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void* in1, int lout, void* out0,
-               void* out1, size_t batch, void* stream, float2* work) {
+               void* out1, size_t batch, void* stream, void* work) {
     const long long n = 1ll << p->corder;
-    float2* tmp[2] = {work, work + (size_t)batch * (size_t)n};
+    void* tmp[2] = {work, static_cast<char*>(work) + (size_t)batch * (size_t)n * 2 * p->ss};
     const void* s0 = in0;
     const void* s1 = in1;
     for (int q = 0; q < p->mp; ++q) {
         const int role = q == 0 ? 0 : (q == p->mp - 1 ? 1 : 2);
         const int lay = role == 0 ? lin : (role == 1 ? lout : zl::AOS);
-        const zl::MacroEntry& e = g_macro[p->logl[q]][role][lay];
+        const zl::MacroEntry& e = (p->f64 ? g_macro64 : g_macro)[p->logl[q]][role][lay];
         if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no HBM-pass kernel for this layout");
         void* d0 = role == 1 ? out0 : tmp[q & 1];
         void* d1 = role == 1 ? out1 : nullptr;
@@ -277,42 +306,64 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
 size_t macro_work_bytes(zlfft_plan p, size_t batch) {
     const size_t n = (size_t)1 << p->corder;
     const size_t bufs = p->mp > 2 ? 2 : 1;
-    size_t b = bufs * batch * n * sizeof(float2);
-    if (p->kind == ZLFFT_KIND_RFFT) b += batch * n * sizeof(float2);  // Z
+    size_t b = bufs * batch * n * 2 * p->ss;
+    if (p->kind == ZLFFT_KIND_RFFT) b += batch * n * 2 * p->ss;  // Z
     return b;
+}
+
+// RFFT post-twiddle / squared magnitude and pre-twiddle around the multi-pass CFFT
+template <class CX>
+int rfft_post(zlfft_plan p, int kind, int lout, const void* z, void* out0, void* out1, size_t batch, void* stream) {
+    const size_t n = (size_t)1 << p->corder;
+    const long long threads = (long long)batch * (long long)(n / 2 + 1);
+    const unsigned grid = (unsigned)((threads + 255) / 256);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const CX* zz = static_cast<const CX*>(z);
+    const CX* rtw = static_cast<const CX*>(p->d_rtw);
+    if (kind == zl::R2C_SQR)
+        zl::rfft_post_kernel<zl::R2C_SQR, zl::AOS, CX><<<grid, 256, 0, s>>>(zz, out0, nullptr, rtw, p->corder,
+                                                                           (long long)batch);
+    else if (lout == zl::AOS)
+        zl::rfft_post_kernel<zl::R2C, zl::AOS, CX><<<grid, 256, 0, s>>>(zz, out0, out1, rtw, p->corder, (long long)batch);
+    else
+        zl::rfft_post_kernel<zl::R2C, zl::SOA, CX><<<grid, 256, 0, s>>>(zz, out0, out1, rtw, p->corder, (long long)batch);
+    ZL_HIP(hipGetLastError());
+    return ZLFFT_OK;
+}
+
+template <class CX>
+int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, size_t batch, void* stream) {
+    const size_t n = (size_t)1 << p->corder;
+    const long long threads = (long long)batch * (long long)(n / 2 + 1);
+    const unsigned grid = (unsigned)((threads + 255) / 256);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    CX* zz = static_cast<CX*>(z);
+    const CX* rtw = static_cast<const CX*>(p->d_rtw);
+    if (lin == zl::AOS)
+        zl::rfft_pre_kernel<zl::AOS, CX><<<grid, 256, 0, s>>>(in0, in1, zz, rtw, p->corder, (long long)batch);
+    else
+        zl::rfft_pre_kernel<zl::SOA, CX><<<grid, 256, 0, s>>>(in0, in1, zz, rtw, p->corder, (long long)batch);
+    ZL_HIP(hipGetLastError());
+    return ZLFFT_OK;
 }
 
 int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
                  size_t batch, int backward, void* stream) {
     if (batch == 0) return ZLFFT_OK;
     if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
-    float2* work = static_cast<float2*>(p->d_work);
+    void* work = p->d_work;
     if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work);
     const size_t n = (size_t)1 << p->corder;
-    float2* z = work + (p->mp > 2 ? 2 : 1) * batch * n;
-    const long long threads = (long long)batch * (long long)(n / 2 + 1);
-    const unsigned grid = (unsigned)((threads + 255) / 256);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    void* z = static_cast<char*>(work) + (p->mp > 2 ? 2 : 1) * batch * n * 2 * p->ss;
     if (kind == zl::R2C || kind == zl::R2C_SQR) {
         if (int rc = macro_cfft(p, 0, zl::AOS, in0, nullptr, zl::AOS, z, nullptr, batch, stream, work)) return rc;
-        if (kind == zl::R2C_SQR)
-            zl::rfft_post_kernel<zl::R2C_SQR, zl::AOS><<<grid, 256, 0, s>>>(z, out0, nullptr, p->d_rtw, p->corder,
-                                                                            (long long)batch);
-        else if (lout == zl::AOS)
-            zl::rfft_post_kernel<zl::R2C, zl::AOS><<<grid, 256, 0, s>>>(z, out0, out1, p->d_rtw, p->corder,
-                                                                        (long long)batch);
-        else
-            zl::rfft_post_kernel<zl::R2C, zl::SOA><<<grid, 256, 0, s>>>(z, out0, out1, p->d_rtw, p->corder,
-                                                                        (long long)batch);
-        ZL_HIP(hipGetLastError());
-        return ZLFFT_OK;
+        return p->f64 ? rfft_post<double2>(p, kind, lout, z, out0, out1, batch, stream)
+                      : rfft_post<float2>(p, kind, lout, z, out0, out1, batch, stream);
     }
     // C2R
-    if (lin == zl::AOS)
-        zl::rfft_pre_kernel<zl::AOS><<<grid, 256, 0, s>>>(in0, in1, z, p->d_rtw, p->corder, (long long)batch);
-    else
-        zl::rfft_pre_kernel<zl::SOA><<<grid, 256, 0, s>>>(in0, in1, z, p->d_rtw, p->corder, (long long)batch);
-    ZL_HIP(hipGetLastError());
+    if (int rc = p->f64 ? rfft_pre<double2>(p, lin, in0, in1, z, batch, stream)
+                        : rfft_pre<float2>(p, lin, in0, in1, z, batch, stream))
+        return rc;
     return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work);
 }
 
@@ -343,7 +394,7 @@ int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, si
 int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
            size_t batch, int backward, void* stream) {
     if (p->macro) return launch_macro(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream);
-    if (p->precision != ZLFFT_PREC_F32) return launch_mfma(p, lin, lout, in0, out0, batch, backward, stream);
+    if (p->precision != ZLFFT_PREC_F32 && !p->f64) return launch_mfma(p, lin, lout, in0, out0, batch, backward, stream);
     const zl::KernelEntry& e = (*p->tab)[kind][lin][lout];
     if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no kernel for this kind/layout combination");
     if (batch == 0) return ZLFFT_OK;
@@ -449,14 +500,16 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     if (!plan) return fail(ZLFFT_ERR_NULL, "plan out-pointer is NULL");
     *plan = nullptr;
     if (kind != ZLFFT_KIND_CFFT && kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "unknown plan kind");
-    if (precision < ZLFFT_PREC_F32 || precision > ZLFFT_PREC_BF16_MFMA_SPLIT3)
+    if (precision < ZLFFT_PREC_F32 || precision > ZLFFT_PREC_F64)
         return fail(ZLFFT_ERR_UNSUPPORTED, "unknown precision");
-    if (precision != ZLFFT_PREC_F32 && (kind != ZLFFT_KIND_CFFT || order != 12))
+    const bool f64 = precision == ZLFFT_PREC_F64;
+    const bool mfma = precision != ZLFFT_PREC_F32 && !f64;
+    if (mfma && (kind != ZLFFT_KIND_CFFT || order != 12))
         return fail(ZLFFT_ERR_UNSUPPORTED, "the MFMA precisions cover the 4096-point CFFT (order 12) only");
     if (kind == ZLFFT_KIND_RFFT && order < 1)
         return fail(ZLFFT_ERR_INVALID_ORDER, "RFFT order must be >= 1 (the reference underflows at 0)");
     const size_t corder = kind == ZLFFT_KIND_RFFT ? order - 1 : order;
-    if (corder > (size_t)zl::kMaxOrder)
+    if (corder > (size_t)(f64 ? zl::kMaxOrder64 : zl::kMaxOrder))
         return fail(ZLFFT_ERR_INVALID_ORDER, "order " + std::to_string(order) + " exceeds the supported maximum");
     std::call_once(g_tab_once, fill_tables);
 
@@ -466,8 +519,26 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     p->precision = precision;
     p->order = (int)order;
     p->corder = (int)corder;
-    p->macro = corder > (size_t)zl::kMaxLdsOrder;
-    p->tab = p->macro ? nullptr : &g_tab[corder];
+    p->f64 = f64;
+    p->ss = f64 ? 8 : 4;
+    p->macro = corder > (size_t)(f64 ? zl::kMaxLdsOrder64 : zl::kMaxLdsOrder);
+    p->tab = p->macro ? nullptr : f64 ? &g_tab64[corder] : &g_tab[corder];
+    const size_t csz = 2 * p->ss;  // bytes per complex value
+    // twiddle generators for the plan's complex type
+    auto gen_pass = [&](void* out, size_t off, int lr, int lw, int cnt) {
+        const unsigned g = (unsigned)((cnt + 255) / 256);
+        if (f64)
+            gen_pass_twiddles<<<g, 256, 0, p->stream>>>(static_cast<double2*>(out) + off, lr, lw, cnt);
+        else
+            gen_pass_twiddles<<<g, 256, 0, p->stream>>>(static_cast<float2*>(out) + off, lr, lw, cnt);
+    };
+    auto gen_macro = [&](void* out, int log_lw, int shift, int cnt) {
+        const unsigned g = (unsigned)((cnt + 255) / 256);
+        if (f64)
+            gen_macro_twiddles<<<g, 256, 0, p->stream>>>(static_cast<double2*>(out), log_lw, shift, cnt);
+        else
+            gen_macro_twiddles<<<g, 256, 0, p->stream>>>(static_cast<float2*>(out), log_lw, shift, cnt);
+    };
     auto cleanup = [&](int rc) {
         zlfft_plan_destroy(p);
         return rc;
@@ -478,21 +549,20 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreateWithFlags"));
 
     if (p->macro) {
-        const zl::MacroPlan mpl = zl::macro_plan((int)corder);
+        const zl::MacroPlan mpl = f64 ? zl::macro_plan64((int)corder) : zl::macro_plan((int)corder);
         p->mp = mpl.np;
         int lw = 0;
         for (int q = 0; q < mpl.np; ++q) {
             p->logl[q] = mpl.logl[q];
             p->logw[q] = lw;
-            const zl::MacroEntry& me = g_macro[mpl.logl[q]][0][0];
+            const zl::MacroEntry& me = (f64 ? g_macro64 : g_macro)[mpl.logl[q]][0][0];
             if (me.tw_total > 0) {
-                e = hipMalloc(&p->d_itw[q], sizeof(float2) * (size_t)me.tw_total);
+                e = hipMalloc(&p->d_itw[q], csz * (size_t)me.tw_total);
                 if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
                 size_t off = 0;
                 for (int r = 1; r < me.np; ++r) {
                     const int cnt = ((1 << me.lr[r]) - 1) << me.lw[r];
-                    gen_pass_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_itw[q] + off, me.lr[r],
-                                                                                  me.lw[r], cnt);
+                    gen_pass(p->d_itw[q], off, me.lr[r], me.lw[r], cnt);
                     off += (size_t)cnt;
                 }
             }
@@ -501,11 +571,11 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                 const int h = (log_lw + 1) / 2;
                 p->logh[q] = h;
                 const int na = 1 << (log_lw - h), nb = 1 << h;
-                e = hipMalloc(&p->d_twa[q], sizeof(float2) * (size_t)na);
-                if (e == hipSuccess) e = hipMalloc(&p->d_twb[q], sizeof(float2) * (size_t)nb);
+                e = hipMalloc(&p->d_twa[q], csz * (size_t)na);
+                if (e == hipSuccess) e = hipMalloc(&p->d_twb[q], csz * (size_t)nb);
                 if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
-                gen_macro_twiddles<<<(na + 255) / 256, 256, 0, p->stream>>>(p->d_twa[q], log_lw, h, na);
-                gen_macro_twiddles<<<(nb + 255) / 256, 256, 0, p->stream>>>(p->d_twb[q], log_lw, 0, nb);
+                gen_macro(p->d_twa[q], log_lw, h, na);
+                gen_macro(p->d_twb[q], log_lw, 0, nb);
             }
             lw += mpl.logl[q];
         }
@@ -526,7 +596,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                     if (e != hipSuccess) return cleanup(hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
                     p->stream_blocks[k][i][o] = nb;
                 }
-        if (precision != ZLFFT_PREC_F32) {
+        if (mfma) {
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&p->mfma_blocks, zl_mfma_kernel(precision, 0, 1), 256, 0);
             if (e != hipSuccess) return cleanup(hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
         }
@@ -549,18 +619,19 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     }
     const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[zl::C2C][0][0];
     if (ent.tw_total > 0) {
-        e = hipMalloc(&p->d_tw, sizeof(float2) * (size_t)ent.tw_total);
+        e = hipMalloc(&p->d_tw, csz * (size_t)ent.tw_total);
         if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
         size_t off = 0;
         for (int q = 1; q < ent.np; ++q) {
             const int cnt = ((1 << ent.lr[q]) - 1) << ent.lw[q];
-            gen_pass_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_tw + off, ent.lr[q], ent.lw[q], cnt);
+            gen_pass(p->d_tw, off, ent.lr[q], ent.lw[q], cnt);
             off += (size_t)cnt;
         }
         if (precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT) {
             e = hipMalloc(&p->d_tw16, sizeof(float2) * (size_t)ent.tw_total);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
-            scale_twiddles16<<<(ent.tw_total + 255) / 256, 256, 0, p->stream>>>(p->d_tw, p->d_tw16, ent.tw_total);
+            scale_twiddles16<<<(ent.tw_total + 255) / 256, 256, 0, p->stream>>>(static_cast<const float2*>(p->d_tw),
+                                                                                p->d_tw16, ent.tw_total);
         }
         e = hipGetLastError();
         if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
@@ -569,9 +640,13 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         const long long nreal = 1ll << order;
         p->n_rtw = (int)(nreal / 4);
         if (p->n_rtw > 0) {
-            e = hipMalloc(&p->d_rtw, sizeof(float2) * (size_t)p->n_rtw);
+            e = hipMalloc(&p->d_rtw, csz * (size_t)p->n_rtw);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "rfft twiddle hipMalloc failed"));
-            gen_rfft_twiddles<<<(p->n_rtw + 255) / 256, 256, 0, p->stream>>>(p->d_rtw, nreal, p->n_rtw);
+            const unsigned g = (unsigned)((p->n_rtw + 255) / 256);
+            if (f64)
+                gen_rfft_twiddles<<<g, 256, 0, p->stream>>>(static_cast<double2*>(p->d_rtw), nreal, p->n_rtw);
+            else
+                gen_rfft_twiddles<<<g, 256, 0, p->stream>>>(static_cast<float2*>(p->d_rtw), nreal, p->n_rtw);
             e = hipGetLastError();
             if (e != hipSuccess) return cleanup(hip_fail(e, "rfft twiddle generation"));
         }
@@ -619,7 +694,7 @@ int zlfft_cfft_exec(zlfft_plan p, int direction, int in_layout, const void* in0,
                   stream);
 }
 
-int zlfft_rfft_forward_exec(zlfft_plan p, const float* in, int out_layout, void* out0, void* out1, size_t batch,
+int zlfft_rfft_forward_exec(zlfft_plan p, const void* in, int out_layout, void* out0, void* out1, size_t batch,
                             void* stream) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
@@ -628,14 +703,14 @@ int zlfft_rfft_forward_exec(zlfft_plan p, const float* in, int out_layout, void*
     return launch(p, zl::R2C, 0, out_layout, in, nullptr, out0, out1, batch, 0, stream);
 }
 
-int zlfft_rfft_forward_sqr_mag_exec(zlfft_plan p, const float* in, float* out, size_t batch, void* stream) {
+int zlfft_rfft_forward_sqr_mag_exec(zlfft_plan p, const void* in, void* out, size_t batch, void* stream) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
     if (batch && (!in || !out)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
     return launch(p, zl::R2C_SQR, 0, 0, in, nullptr, out, nullptr, batch, 0, stream);
 }
 
-int zlfft_rfft_backward_exec(zlfft_plan p, int in_layout, const void* in0, const void* in1, float* out, size_t batch,
+int zlfft_rfft_backward_exec(zlfft_plan p, int in_layout, const void* in0, const void* in1, void* out, size_t batch,
                              void* stream) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
@@ -651,53 +726,53 @@ int zlfft_cfft_execute(zlfft_plan p, int direction, int in_layout, const void* i
     if (!valid_layout(in_layout) || !valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
     if (!in0 || !out0 || (in_layout == ZLFFT_SOA && !in1) || (out_layout == ZLFFT_SOA && !out1))
         return fail(ZLFFT_ERR_NULL, "NULL data pointer");
-    const size_t n = (size_t)1 << p->order;
-    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 8 * n : 4 * n,
-                 in_layout == ZLFFT_SOA ? 4 * n : 0};
-    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 8 * n : 4 * n,
-                 out_layout == ZLFFT_SOA ? 4 * n : 0};
+    const size_t n = (size_t)1 << p->order, ss = p->ss;
+    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 2 * ss * n : ss * n,
+                 in_layout == ZLFFT_SOA ? ss * n : 0};
+    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 2 * ss * n : ss * n,
+                 out_layout == ZLFFT_SOA ? ss * n : 0};
     return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void* o1, hipStream_t s) {
         return zlfft_cfft_exec(p, direction, in_layout, i0, i1, out_layout, o0, o1, 1, s);
     });
 }
 
-int zlfft_rfft_forward(zlfft_plan p, const float* in, int out_layout, void* out0, void* out1) {
+int zlfft_rfft_forward(zlfft_plan p, const void* in, int out_layout, void* out0, void* out1) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
     if (!valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
     if (!in || !out0 || (out_layout == ZLFFT_SOA && !out1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
-    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1;
-    const Buf bi{in, nullptr, 4 * n, 0};
-    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 8 * nb : 4 * nb,
-                 out_layout == ZLFFT_SOA ? 4 * nb : 0};
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
+    const Buf bi{in, nullptr, ss * n, 0};
+    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 2 * ss * nb : ss * nb,
+                 out_layout == ZLFFT_SOA ? ss * nb : 0};
     return run_sync(p, bi, bo, [&](const void* i0, const void*, void* o0, void* o1, hipStream_t s) {
-        return zlfft_rfft_forward_exec(p, static_cast<const float*>(i0), out_layout, o0, o1, 1, s);
+        return zlfft_rfft_forward_exec(p, i0, out_layout, o0, o1, 1, s);
     });
 }
 
-int zlfft_rfft_forward_sqr_mag(zlfft_plan p, const float* in, float* out) {
+int zlfft_rfft_forward_sqr_mag(zlfft_plan p, const void* in, void* out) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
     if (!in || !out) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
-    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1;
-    const Buf bi{in, nullptr, 4 * n, 0};
-    const Buf bo{out, nullptr, 4 * nb, 0};
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
+    const Buf bi{in, nullptr, ss * n, 0};
+    const Buf bo{out, nullptr, ss * nb, 0};
     return run_sync(p, bi, bo, [&](const void* i0, const void*, void* o0, void*, hipStream_t s) {
-        return zlfft_rfft_forward_sqr_mag_exec(p, static_cast<const float*>(i0), static_cast<float*>(o0), 1, s);
+        return zlfft_rfft_forward_sqr_mag_exec(p, i0, o0, 1, s);
     });
 }
 
-int zlfft_rfft_backward(zlfft_plan p, int in_layout, const void* in0, const void* in1, float* out) {
+int zlfft_rfft_backward(zlfft_plan p, int in_layout, const void* in0, const void* in1, void* out) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
     if (!valid_layout(in_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
     if (!in0 || !out || (in_layout == ZLFFT_SOA && !in1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
-    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1;
-    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 8 * nb : 4 * nb,
-                 in_layout == ZLFFT_SOA ? 4 * nb : 0};
-    const Buf bo{out, nullptr, 4 * n, 0};
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
+    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 2 * ss * nb : ss * nb,
+                 in_layout == ZLFFT_SOA ? ss * nb : 0};
+    const Buf bo{out, nullptr, ss * n, 0};
     return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void*, hipStream_t s) {
-        return zlfft_rfft_backward_exec(p, in_layout, i0, i1, static_cast<float*>(o0), 1, s);
+        return zlfft_rfft_backward_exec(p, in_layout, i0, i1, o0, 1, s);
     });
 }
 
@@ -705,6 +780,7 @@ int zlfft_plan_copy_rfft_twiddles(zlfft_plan p, float* host_out, size_t n_floats
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
     if (!host_out) return fail(ZLFFT_ERR_NULL, "host_out is NULL");
+    if (p->f64) return fail(ZLFFT_ERR_UNSUPPORTED, "FP64 plans hold double twiddles; this call copies floats");
     const size_t need = 2 * (size_t)p->n_rtw;
     if (n_floats < need) return fail(ZLFFT_ERR_INVALID_ARG, "buffer too small");
     if (need) ZL_HIP(hipMemcpy(host_out, p->d_rtw, need * sizeof(float), hipMemcpyDeviceToHost));

@@ -61,32 +61,59 @@ struct Geo {
 
 // ---------------------------------------------------------------- codelets
 
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-// a * (-i)
-__device__ __forceinline__ float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }
-__device__ __forceinline__ float2 cmul(float2 a, float2 w) {
-    return make_float2(__builtin_fmaf(-a.y, w.y, a.x * w.x), __builtin_fmaf(a.y, w.x, a.x * w.y));
-}
-__device__ __forceinline__ float2 swapri(float2 a) { return make_float2(a.y, a.x); }
+// Complex element types: float2 (the FP32 path) and double2 (the FP64 path,
+// CFFT<double> / RFFT<double>).  Every codelet, pass and the LDS, multi-pass
+// and RFFT kernels are written once over CX; the FP32 instantiations compile
+// to the same operations as FP32-only code.
+template <class CX>
+struct CxT;
+template <>
+struct CxT<float2> {
+    using S = float;
+    static __device__ __forceinline__ float2 mk(float x, float y) { return make_float2(x, y); }
+    static __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    static constexpr float kH = 0.70710678118654752440f;    // 1/sqrt(2)
+    static constexpr float kC16 = 0.92387953251128675613f;  // cos(pi/8)
+    static constexpr float kS16 = 0.38268343236508977173f;  // sin(pi/8)
+};
+template <>
+struct CxT<double2> {
+    using S = double;
+    static __device__ __forceinline__ double2 mk(double x, double y) { return make_double2(x, y); }
+    static __device__ __forceinline__ double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+    static constexpr double kH = 0.70710678118654752440;
+    static constexpr double kC16 = 0.92387953251128675613;
+    static constexpr double kS16 = 0.38268343236508977173;
+};
 
-constexpr float kH = 0.70710678118654752440f;  // 1/sqrt(2)
-constexpr float kC16 = 0.92387953251128675613f; // cos(pi/8)
-constexpr float kS16 = 0.38268343236508977173f; // sin(pi/8)
+template <class CX>
+__device__ __forceinline__ CX cadd(CX a, CX b) { return CxT<CX>::mk(a.x + b.x, a.y + b.y); }
+template <class CX>
+__device__ __forceinline__ CX csub(CX a, CX b) { return CxT<CX>::mk(a.x - b.x, a.y - b.y); }
+// a * (-i)
+template <class CX>
+__device__ __forceinline__ CX mul_mi(CX a) { return CxT<CX>::mk(a.y, -a.x); }
+template <class CX>
+__device__ __forceinline__ CX cmul(CX a, CX w) {
+    return CxT<CX>::mk(CxT<CX>::fma(-a.y, w.y, a.x * w.x), CxT<CX>::fma(a.y, w.x, a.x * w.y));
+}
+template <class CX>
+__device__ __forceinline__ CX swapri(CX a) { return CxT<CX>::mk(a.y, a.x); }
 
 // x * W_16^e for the exponents a radix-16 codelet needs
-template <int E>
-__device__ __forceinline__ float2 w16(float2 a) {
+template <int E, class CX>
+__device__ __forceinline__ CX w16(CX a) {
+    using T = CxT<CX>;
     constexpr int e = E & 15;
     if constexpr (e == 0) return a;
     else if constexpr (e == 4) return mul_mi(a);
-    else if constexpr (e == 8) return make_float2(-a.x, -a.y);
-    else if constexpr (e == 12) return make_float2(-a.y, a.x);
-    else if constexpr (e == 2) return make_float2((a.x + a.y) * kH, (a.y - a.x) * kH);
-    else if constexpr (e == 6) return make_float2((a.y - a.x) * kH, (-(a.x + a.y)) * kH);
-    else if constexpr (e == 1) return cmul(a, make_float2(kC16, -kS16));
-    else if constexpr (e == 3) return cmul(a, make_float2(kS16, -kC16));
-    else if constexpr (e == 9) return cmul(a, make_float2(-kC16, kS16));
+    else if constexpr (e == 8) return T::mk(-a.x, -a.y);
+    else if constexpr (e == 12) return T::mk(-a.y, a.x);
+    else if constexpr (e == 2) return T::mk((a.x + a.y) * T::kH, (a.y - a.x) * T::kH);
+    else if constexpr (e == 6) return T::mk((a.y - a.x) * T::kH, (-(a.x + a.y)) * T::kH);
+    else if constexpr (e == 1) return cmul(a, T::mk(T::kC16, -T::kS16));
+    else if constexpr (e == 3) return cmul(a, T::mk(T::kS16, -T::kC16));
+    else if constexpr (e == 9) return cmul(a, T::mk(-T::kC16, T::kS16));
     else {
         static_assert(e == 0, "unsupported W16 exponent");
         return a;
@@ -94,78 +121,68 @@ __device__ __forceinline__ float2 w16(float2 a) {
 }
 
 // In-place forward DFT-r on x[0..r), natural-order output.
-template <int r>
-__device__ __forceinline__ void dft(float2* x);
-
-template <>
-__device__ __forceinline__ void dft<1>(float2*) {}
-
-template <>
-__device__ __forceinline__ void dft<2>(float2* x) {
-    const float2 a = x[0], b = x[1];
-    x[0] = cadd(a, b);
-    x[1] = csub(a, b);
-}
-
-// radix-4 butterfly in the reference's operation order
-// (reference src/common/zldsp_fft_common_radix.hpp:184-235)
-template <>
-__device__ __forceinline__ void dft<4>(float2* x) {
-    const float2 t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
-    const float2 t2 = cadd(x[1], x[3]), t3 = csub(x[1], x[3]);
-    x[0] = cadd(t0, t2);
-    x[2] = csub(t0, t2);
-    x[1] = make_float2(t1.x + t3.y, t1.y - t3.x);
-    x[3] = make_float2(t1.x - t3.y, t1.y + t3.x);
-}
-
-// radix-8 as two radix-4 plus W_8 via 1/sqrt(2)
-// (reference src/common/zldsp_fft_common_radix.hpp:499-560)
-template <>
-__device__ __forceinline__ void dft<8>(float2* x) {
-    float2 e[4] = {x[0], x[2], x[4], x[6]};
-    float2 o[4] = {x[1], x[3], x[5], x[7]};
-    dft<4>(e);
-    dft<4>(o);
-    o[1] = w16<2>(o[1]);
-    o[2] = w16<4>(o[2]);
-    o[3] = w16<6>(o[3]);
+//  r = 4: radix-4 butterfly in the reference's operation order
+//         (reference src/common/zldsp_fft_common_radix.hpp:184-235)
+//  r = 8: two radix-4 plus W_8 via 1/sqrt(2)
+//         (reference src/common/zldsp_fft_common_radix.hpp:499-560)
+//  r = 16: 4 x 4: DFT-4 over x[j + 4m], twiddle W16^{j q}, DFT-4 over j.
+template <int r, class CX>
+__device__ __forceinline__ void dft(CX* x) {
+    if constexpr (r == 1) {
+    } else if constexpr (r == 2) {
+        const CX a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (r == 4) {
+        const CX t0 = cadd(x[0], x[2]), t1 = csub(x[0], x[2]);
+        const CX t2 = cadd(x[1], x[3]), t3 = csub(x[1], x[3]);
+        x[0] = cadd(t0, t2);
+        x[2] = csub(t0, t2);
+        x[1] = CxT<CX>::mk(t1.x + t3.y, t1.y - t3.x);
+        x[3] = CxT<CX>::mk(t1.x - t3.y, t1.y + t3.x);
+    } else if constexpr (r == 8) {
+        CX e[4] = {x[0], x[2], x[4], x[6]};
+        CX o[4] = {x[1], x[3], x[5], x[7]};
+        dft<4>(e);
+        dft<4>(o);
+        o[1] = w16<2>(o[1]);
+        o[2] = w16<4>(o[2]);
+        o[3] = w16<6>(o[3]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        x[q] = cadd(e[q], o[q]);
-        x[q + 4] = csub(e[q], o[q]);
-    }
-}
-
-// radix-16 as 4 x 4: DFT-4 over x[j + 4m], twiddle W16^{j q}, DFT-4 over j.
-template <>
-__device__ __forceinline__ void dft<16>(float2* x) {
-    float2 b[4][4];
+        for (int q = 0; q < 4; ++q) {
+            x[q] = cadd(e[q], o[q]);
+            x[q + 4] = csub(e[q], o[q]);
+        }
+    } else if constexpr (r == 16) {
+        CX b[4][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        b[j][0] = x[j];
-        b[j][1] = x[j + 4];
-        b[j][2] = x[j + 8];
-        b[j][3] = x[j + 12];
-        dft<4>(b[j]);
-    }
-    b[1][1] = w16<1>(b[1][1]);
-    b[1][2] = w16<2>(b[1][2]);
-    b[1][3] = w16<3>(b[1][3]);
-    b[2][1] = w16<2>(b[2][1]);
-    b[2][2] = w16<4>(b[2][2]);
-    b[2][3] = w16<6>(b[2][3]);
-    b[3][1] = w16<3>(b[3][1]);
-    b[3][2] = w16<6>(b[3][2]);
-    b[3][3] = w16<9>(b[3][3]);
+        for (int j = 0; j < 4; ++j) {
+            b[j][0] = x[j];
+            b[j][1] = x[j + 4];
+            b[j][2] = x[j + 8];
+            b[j][3] = x[j + 12];
+            dft<4>(b[j]);
+        }
+        b[1][1] = w16<1>(b[1][1]);
+        b[1][2] = w16<2>(b[1][2]);
+        b[1][3] = w16<3>(b[1][3]);
+        b[2][1] = w16<2>(b[2][1]);
+        b[2][2] = w16<4>(b[2][2]);
+        b[2][3] = w16<6>(b[2][3]);
+        b[3][1] = w16<3>(b[3][1]);
+        b[3][2] = w16<6>(b[3][2]);
+        b[3][3] = w16<9>(b[3][3]);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float2 c[4] = {b[0][q], b[1][q], b[2][q], b[3][q]};
-        dft<4>(c);
-        x[q] = c[0];
-        x[q + 4] = c[1];
-        x[q + 8] = c[2];
-        x[q + 12] = c[3];
+        for (int q = 0; q < 4; ++q) {
+            CX c[4] = {b[0][q], b[1][q], b[2][q], b[3][q]};
+            dft<4>(c);
+            x[q] = c[0];
+            x[q + 4] = c[1];
+            x[q + 8] = c[2];
+            x[q + 12] = c[3];
+        }
+    } else {
+        static_assert(r == 1, "unsupported radix");
     }
 }
 
@@ -173,19 +190,19 @@ __device__ __forceinline__ void dft<16>(float2* x) {
 
 // Pass P of geometry G on registers v; slds is this transform's LDS row.
 // Non-last passes leave the next pass's inputs in v (exchanged through LDS).
-template <class G, int P>
-__device__ __forceinline__ void run_pass(float2 (&v)[G::R], float2* slds, const float2* __restrict__ tw, int t) {
+template <class G, int P, class CX>
+__device__ __forceinline__ void run_pass(CX (&v)[G::R], CX* slds, const CX* __restrict__ tw, int t) {
     constexpr int lr = G::lr(P), r = 1 << lr, nb = G::R / r, lw = G::lw(P), w = 1 << lw;
     constexpr bool last = (P == G::NP - 1);
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
-        float2 x[r];
+        CX x[r];
 #pragma unroll
         for (int m = 0; m < r; ++m) x[m] = v[b + m * nb];
         if constexpr (P > 0) {
             const int i = t + G::T * b;
             const int k = i & (w - 1);
-            const float2* twp = tw + G::tw_off(P) + k;
+            const CX* twp = tw + G::tw_off(P) + k;
 #pragma unroll
             for (int m = 1; m < r; ++m) x[m] = cmul(x[m], twp[(m - 1) * w]);
         }
@@ -209,8 +226,8 @@ __device__ __forceinline__ void run_pass(float2 (&v)[G::R], float2* slds, const 
     }
 }
 
-template <class G>
-__device__ __forceinline__ void run_passes(float2 (&v)[G::R], float2* slds, const float2* __restrict__ tw, int t) {
+template <class G, class CX>
+__device__ __forceinline__ void run_passes(CX (&v)[G::R], CX* slds, const CX* __restrict__ tw, int t) {
     if constexpr (G::NP > 0) run_pass<G, 0>(v, slds, tw, t);
 }
 
@@ -221,8 +238,8 @@ struct KArgs {
     const void* in1;   // SoA: imag part
     void* out0;
     void* out1;
-    const float2* tw;  // Stockham pass twiddles of this geometry
-    const float2* rtw; // RFFT (wc, ws) for k = 1..M/2, at [k - 1]
+    const void* tw;    // Stockham pass twiddles of this geometry (complex type of the kernel)
+    const void* rtw;   // RFFT (wc, ws) for k = 1..M/2, at [k - 1]
     long long batch;
     int backward;      // C2C only
     float2* sink;      // kSinkBytes of scratch that inactive lanes of the
@@ -232,16 +249,20 @@ struct KArgs {
 // size of KArgs::sink: two arrays of the largest LDS transform (2^14 complex)
 constexpr long long kSinkBytes = 2ll * 8 * (1 << 14);
 
-__device__ __forceinline__ float2 ld_c(const void* p0, const void* p1, int lay, long long e) {
-    if (lay == AOS) return reinterpret_cast<const float2*>(p0)[e];
-    return make_float2(reinterpret_cast<const float*>(p0)[e], reinterpret_cast<const float*>(p1)[e]);
+template <class CX = float2>
+__device__ __forceinline__ CX ld_c(const void* p0, const void* p1, int lay, long long e) {
+    using S = typename CxT<CX>::S;
+    if (lay == AOS) return reinterpret_cast<const CX*>(p0)[e];
+    return CxT<CX>::mk(reinterpret_cast<const S*>(p0)[e], reinterpret_cast<const S*>(p1)[e]);
 }
-__device__ __forceinline__ void st_c(void* p0, void* p1, int lay, long long e, float2 v) {
+template <class CX>
+__device__ __forceinline__ void st_c(void* p0, void* p1, int lay, long long e, CX v) {
+    using S = typename CxT<CX>::S;
     if (lay == AOS) {
-        reinterpret_cast<float2*>(p0)[e] = v;
+        reinterpret_cast<CX*>(p0)[e] = v;
     } else {
-        reinterpret_cast<float*>(p0)[e] = v.x;
-        reinterpret_cast<float*>(p1)[e] = v.y;
+        reinterpret_cast<S*>(p0)[e] = v.x;
+        reinterpret_cast<S*>(p1)[e] = v.y;
     }
 }
 
@@ -254,23 +275,25 @@ __device__ __forceinline__ void st_c(void* p0, void* p1, int lay, long long e, f
 //   C2R    : M+1 bins in layout LIN -> N real
 //            (reference RFFT::backward, src/zldsp_fft_rfft.hpp:86-97, 132-138)
 // For the real kinds G describes the inner CFFT of size M = N/2.
-template <class G, int KIND, int LIN, int LOUT>
+template <class G, int KIND, int LIN, int LOUT, class CX = float2>
 __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
-    __shared__ float2 lds[G::TPW * G::LDSN];
+    using T = CxT<CX>;
+    using S = typename T::S;
+    __shared__ CX lds[G::TPW * G::LDSN];
     const int tid = threadIdx.x;
     const int slot = tid / G::T;
     const int t = tid - slot * G::T;
     const long long f = (long long)blockIdx.x * G::TPW + slot;
     const bool active = f < a.batch;
-    float2* slds = lds + slot * G::LDSN;
+    CX* slds = lds + slot * G::LDSN;
     constexpr int M = G::N;  // CFFT size
-    constexpr int ELEM_IN = (KIND == C2C && LIN == SOA) ? 4 : 8;
-    constexpr int ELEM_OUT = (KIND == C2C && LOUT == SOA) ? 4 : 8;
+    constexpr int ELEM_IN = (KIND == C2C && LIN == SOA) ? (int)sizeof(S) : (int)sizeof(CX);
+    constexpr int ELEM_OUT = (KIND == C2C && LOUT == SOA) ? (int)sizeof(S) : (int)sizeof(CX);
     constexpr bool direct_in = G::T * ELEM_IN >= 128 || G::TPW == 1;
     constexpr bool direct_out = G::T * ELEM_OUT >= 128 || G::TPW == 1;
     const bool bwd = (KIND == C2C) ? (a.backward != 0) : (KIND == C2R);
 
-    float2 v[G::R];
+    CX v[G::R];
 
     // ------------------------------------------------------------ input
     if constexpr (KIND == C2R) {
@@ -279,23 +302,23 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
         const long long fb = f * (M + 1);
         if (active) {
             if (t == 0) {
-                const float r0 = ld_c(a.in0, a.in1, LIN, fb).x;
-                const float rm = ld_c(a.in0, a.in1, LIN, fb + M).x;
-                slds[0] = swapri(make_float2((r0 + rm) * 0.5f, (r0 - rm) * 0.5f));
+                const S r0 = ld_c<CX>(a.in0, a.in1, LIN, fb).x;
+                const S rm = ld_c<CX>(a.in0, a.in1, LIN, fb + M).x;
+                slds[0] = swapri(T::mk((r0 + rm) * S(0.5), (r0 - rm) * S(0.5)));
             }
             for (int k = 1 + t; k <= M / 2; k += G::T) {
-                const float2 x1 = ld_c(a.in0, a.in1, LIN, fb + k);
-                const float2 x2 = ld_c(a.in0, a.in1, LIN, fb + (M - k));
-                const float2 w = a.rtw[k - 1];  // (wc, ws)
-                const float si = x1.y + x2.y, dr = x1.x - x2.x;
-                const float zr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(si, w.x, x2.x));
-                const float zi_m = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x2.y));
+                const CX x1 = ld_c<CX>(a.in0, a.in1, LIN, fb + k);
+                const CX x2 = ld_c<CX>(a.in0, a.in1, LIN, fb + (M - k));
+                const CX w = static_cast<const CX*>(a.rtw)[k - 1];  // (wc, ws)
+                const S si = x1.y + x2.y, dr = x1.x - x2.x;
+                const S zr_m = T::fma(dr, w.y, T::fma(si, w.x, x2.x));
+                const S zi_m = T::fma(dr, w.x, T::fma(-si, w.y, x2.y));
                 if (k != M - k) {
-                    const float zr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(-si, w.x, x1.x));
-                    const float zi_k = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x1.y));
-                    slds[G::pad(k)] = make_float2(zi_k, zr_k);
+                    const S zr_k = T::fma(-dr, w.y, T::fma(-si, w.x, x1.x));
+                    const S zi_k = T::fma(dr, w.x, T::fma(-si, w.y, x1.y));
+                    slds[G::pad(k)] = T::mk(zi_k, zr_k);
                 }
-                slds[G::pad(M - k)] = make_float2(zi_m, zr_m);
+                slds[G::pad(M - k)] = T::mk(zi_m, zr_m);
             }
         }
         __syncthreads();
@@ -307,10 +330,10 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
             const long long fb = f * M;
             if (active) {
 #pragma unroll
-                for (int j = 0; j < G::R; ++j) v[j] = ld_c(a.in0, a.in1, lin, fb + t + G::T * j);
+                for (int j = 0; j < G::R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, fb + t + G::T * j);
             } else {
 #pragma unroll
-                for (int j = 0; j < G::R; ++j) v[j] = make_float2(0.f, 0.f);
+                for (int j = 0; j < G::R; ++j) v[j] = T::mk(S(0), S(0));
             }
         } else {
             // coalesced cooperative copy of the workgroup's contiguous block
@@ -320,7 +343,7 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
             for (int j = 0; j < G::R; ++j) {
                 const int e = tid + G::WG * j;
                 const int s = e / M, pos = e - s * M;
-                lds[s * G::LDSN + G::pad(pos)] = (e < wn) ? ld_c(a.in0, a.in1, lin, wb + e) : make_float2(0.f, 0.f);
+                lds[s * G::LDSN + G::pad(pos)] = (e < wn) ? ld_c<CX>(a.in0, a.in1, lin, wb + e) : T::mk(S(0), S(0));
             }
             __syncthreads();
 #pragma unroll
@@ -333,7 +356,7 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
     }
 
     // ------------------------------------------------------------ passes
-    run_passes<G>(v, slds, a.tw, t);
+    run_passes<G>(v, slds, static_cast<const CX*>(a.tw), t);
 
     // ------------------------------------------------------------ output
     if constexpr (KIND == R2C || KIND == R2C_SQR) {
@@ -345,33 +368,33 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
         if (active) {
             const long long fb = f * (M + 1);
             if (t == 0) {
-                const float2 z0 = slds[0];
-                const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+                const CX z0 = slds[0];
+                const S dc = z0.x + z0.y, ny = z0.x - z0.y;
                 if constexpr (KIND == R2C) {
-                    st_c(a.out0, a.out1, LOUT, fb, make_float2(dc, 0.f));
-                    st_c(a.out0, a.out1, LOUT, fb + M, make_float2(ny, 0.f));
+                    st_c(a.out0, a.out1, LOUT, fb, T::mk(dc, S(0)));
+                    st_c(a.out0, a.out1, LOUT, fb + M, T::mk(ny, S(0)));
                 } else {
-                    float* o = reinterpret_cast<float*>(a.out0);
+                    S* o = reinterpret_cast<S*>(a.out0);
                     o[fb] = dc * dc;
                     o[fb + M] = ny * ny;
                 }
             }
             for (int k = 1 + t; k <= M / 2; k += G::T) {
-                const float2 z1 = slds[G::pad(k)];
-                const float2 z2 = slds[G::pad(M - k)];
-                const float2 w = a.rtw[k - 1];
-                const float si = z1.y + z2.y, dr = z1.x - z2.x;
-                const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
-                const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
-                const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
-                const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
+                const CX z1 = slds[G::pad(k)];
+                const CX z2 = slds[G::pad(M - k)];
+                const CX w = static_cast<const CX*>(a.rtw)[k - 1];
+                const S si = z1.y + z2.y, dr = z1.x - z2.x;
+                const S xr_m = T::fma(dr, w.y, T::fma(-si, w.x, z2.x));
+                const S xi_m = T::fma(-dr, w.x, T::fma(-si, w.y, z2.y));
+                const S xr_k = T::fma(-dr, w.y, T::fma(si, w.x, z1.x));
+                const S xi_k = T::fma(-dr, w.x, T::fma(-si, w.y, z1.y));
                 if constexpr (KIND == R2C) {
-                    if (k != M - k) st_c(a.out0, a.out1, LOUT, fb + k, make_float2(xr_k, xi_k));
-                    st_c(a.out0, a.out1, LOUT, fb + (M - k), make_float2(xr_m, xi_m));
+                    if (k != M - k) st_c(a.out0, a.out1, LOUT, fb + k, T::mk(xr_k, xi_k));
+                    st_c(a.out0, a.out1, LOUT, fb + (M - k), T::mk(xr_m, xi_m));
                 } else {
-                    float* o = reinterpret_cast<float*>(a.out0);
-                    if (k != M - k) o[fb + k] = __builtin_fmaf(xr_k, xr_k, xi_k * xi_k);
-                    o[fb + (M - k)] = __builtin_fmaf(xr_m, xr_m, xi_m * xi_m);
+                    S* o = reinterpret_cast<S*>(a.out0);
+                    if (k != M - k) o[fb + k] = T::fma(xr_k, xr_k, xi_k * xi_k);
+                    o[fb + (M - k)] = T::fma(xr_m, xr_m, xi_m * xi_m);
                 }
             }
         }
@@ -524,9 +547,9 @@ fft_stream_kernel(KArgs a) {
     constexpr bool bwd = (KIND == C2C) && BWD != 0;  // direction fixed per instantiation
     const long long ngroups = (a.batch + G::TPW - 1) / G::TPW;
 
-    for (int e = tid; e < TI::total(); e += G::WG) ltw[e] = a.tw[e];
+    for (int e = tid; e < TI::total(); e += G::WG) ltw[e] = static_cast<const float2*>(a.tw)[e];
     if constexpr (KIND != C2C) {
-        for (int e = tid; e < M / 2; e += G::WG) lrtw[e] = a.rtw[e];
+        for (int e = tid; e < M / 2; e += G::WG) lrtw[e] = static_cast<const float2*>(a.rtw)[e];
     }
 
     // Lanes without a transform (past the batch end, or past the last group)
@@ -629,9 +652,10 @@ struct MArgs {
     const void* in1;
     void* out0;
     void* out1;
-    const float2* tw;   // inner Stockham twiddles of the radix-L sub-FFT (geometry GL)
-    const float2* twa;  // macro twiddle table A[a] = W_{L w}^{a 2^h}
-    const float2* twb;  // macro twiddle table B[b] = W_{L w}^{b}, b < 2^h
+    const void* tw;     // inner Stockham twiddles of the radix-L sub-FFT (geometry GL)
+    const void* twa;    // macro twiddle table A[a] = W_{L w}^{a 2^h}
+    const void* twb;    // macro twiddle table B[b] = W_{L w}^{b}, b < 2^h
+                        // (all three of the kernel's complex type)
     long long batch;
     int logn;           // log2 N of the whole transform
     int logw;           // log2 width w of this pass (product of earlier radices)
@@ -648,10 +672,12 @@ struct MArgs {
 // FIRST: reads the user layout LIN (w = 1, no macro twiddle) and writes its
 // contiguous output block through LDS; LAST: writes the user layout LOUT.
 // Intermediate buffers are AoS float2.
-template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT>
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2>
 __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
-    __shared__ float2 lds[C * GL::LDSN];
-    constexpr int L = GL::N, T = GL::T, R = GL::R, WG = C * T;
+    const CX* twa = static_cast<const CX*>(a.twa);
+    const CX* twb = static_cast<const CX*>(a.twb);
+    __shared__ CX lds[C * GL::LDSN];
+    constexpr int L = GL::N, TT = GL::T, R = GL::R, WG = C * TT;
     const int tid = threadIdx.x;
     const int c = tid % C;
     const int t = tid / C;
@@ -664,11 +690,11 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     const long long base = b * n;
     constexpr int lin = FIRST ? LIN : AOS;
     constexpr int lout = LAST ? LOUT : AOS;
-    float2* slds = lds + c * GL::LDSN;
+    CX* slds = lds + c * GL::LDSN;
 
-    float2 v[R];
+    CX v[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = ld_c(a.in0, a.in1, lin, base + i + (long long)(t + T * j) * cols);
+    for (int j = 0; j < R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, base + i + (long long)(t + TT * j) * cols);
     if (FIRST && a.backward) {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
@@ -680,13 +706,13 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         const long long hmask = (1ll << a.logh) - 1;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            const long long m = t + T * j;
+            const long long m = t + TT * j;
             const long long e = (m * k) & emask;
-            const float2 w = cmul(a.twa[e >> a.logh], a.twb[e & hmask]);
+            const CX w = cmul(twa[e >> a.logh], twb[e & hmask]);
             v[j] = cmul(v[j], w);
         }
     }
-    run_passes<GL>(v, slds, a.tw, t);  // v[j] = y_q with q = t + T j
+    run_passes<GL>(v, slds, static_cast<const CX*>(a.tw), t);  // v[j] = y_q with q = t + T j
     if (LAST && a.backward) {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
@@ -695,7 +721,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         // w = 1: out[i L + q]; the workgroup's output block [i0 L, (i0 + C) L) is contiguous
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < R; ++j) slds[GL::pad(t + T * j)] = v[j];
+        for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
         __syncthreads();
         const long long ob = base + i0 * L;
 #pragma unroll
@@ -708,7 +734,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         const long long w = 1ll << a.logw;
         const long long ob = base + ((i >> a.logw) << (a.logw + GL::logn)) + (i & (w - 1));
 #pragma unroll
-        for (int j = 0; j < R; ++j) st_c(a.out0, a.out1, lout, ob + (long long)(t + T * j) * w, v[j]);
+        for (int j = 0; j < R; ++j) st_c(a.out0, a.out1, lout, ob + (long long)(t + TT * j) * w, v[j]);
     }
 }
 
@@ -717,78 +743,82 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
 // thread per pair (k, M - k), k = 0..M/2; k = 0 writes DC and Nyquist.
 // Same formulas as the LDS kernel's fused epilogue
 // (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552).
-template <int KIND, int LOUT>
-__global__ void __launch_bounds__(256) rfft_post_kernel(const float2* __restrict__ z, void* out0, void* out1,
-                                                        const float2* __restrict__ rtw, int logm, long long batch) {
+template <int KIND, int LOUT, class CX = float2>
+__global__ void __launch_bounds__(256) rfft_post_kernel(const CX* __restrict__ z, void* out0, void* out1,
+                                                        const CX* __restrict__ rtw, int logm, long long batch) {
+    using T = CxT<CX>;
+    using S = typename T::S;
     const long long m = 1ll << logm;
     const long long half = m >> 1;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long f = g / (half + 1);
     const long long k = g - f * (half + 1);
     if (f >= batch) return;
-    const float2* zf = z + f * m;
+    const CX* zf = z + f * m;
     const long long fb = f * (m + 1);
     if (k == 0) {
-        const float2 z0 = zf[0];
-        const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+        const CX z0 = zf[0];
+        const S dc = z0.x + z0.y, ny = z0.x - z0.y;
         if constexpr (KIND == R2C) {
-            st_c(out0, out1, LOUT, fb, make_float2(dc, 0.f));
-            st_c(out0, out1, LOUT, fb + m, make_float2(ny, 0.f));
+            st_c(out0, out1, LOUT, fb, T::mk(dc, S(0)));
+            st_c(out0, out1, LOUT, fb + m, T::mk(ny, S(0)));
         } else {
-            reinterpret_cast<float*>(out0)[fb] = dc * dc;
-            reinterpret_cast<float*>(out0)[fb + m] = ny * ny;
+            reinterpret_cast<S*>(out0)[fb] = dc * dc;
+            reinterpret_cast<S*>(out0)[fb + m] = ny * ny;
         }
         return;
     }
-    const float2 z1 = zf[k], z2 = zf[m - k];
-    const float2 w = rtw[k - 1];
-    const float si = z1.y + z2.y, dr = z1.x - z2.x;
-    const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
-    const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
-    const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
-    const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
+    const CX z1 = zf[k], z2 = zf[m - k];
+    const CX w = rtw[k - 1];
+    const S si = z1.y + z2.y, dr = z1.x - z2.x;
+    const S xr_m = T::fma(dr, w.y, T::fma(-si, w.x, z2.x));
+    const S xi_m = T::fma(-dr, w.x, T::fma(-si, w.y, z2.y));
+    const S xr_k = T::fma(-dr, w.y, T::fma(si, w.x, z1.x));
+    const S xi_k = T::fma(-dr, w.x, T::fma(-si, w.y, z1.y));
     if constexpr (KIND == R2C) {
-        if (k != m - k) st_c(out0, out1, LOUT, fb + k, make_float2(xr_k, xi_k));
-        st_c(out0, out1, LOUT, fb + (m - k), make_float2(xr_m, xi_m));
+        if (k != m - k) st_c(out0, out1, LOUT, fb + k, T::mk(xr_k, xi_k));
+        st_c(out0, out1, LOUT, fb + (m - k), T::mk(xr_m, xi_m));
     } else {
-        float* o = reinterpret_cast<float*>(out0);
-        if (k != m - k) o[fb + k] = __builtin_fmaf(xr_k, xr_k, xi_k * xi_k);
-        o[fb + (m - k)] = __builtin_fmaf(xr_m, xr_m, xi_m * xi_m);
+        S* o = reinterpret_cast<S*>(out0);
+        if (k != m - k) o[fb + k] = T::fma(xr_k, xr_k, xi_k * xi_k);
+        o[fb + (m - k)] = T::fma(xr_m, xr_m, xi_m * xi_m);
     }
 }
 
 // RFFT pre-twiddle for the multi-pass sizes: bins (layout LIN, M+1 per frame)
 // -> Z (AoS, M per frame), the input of the backward CFFT
 // (reference src/common/zldsp_fft_common_execute.hpp:398-468).
-template <int LIN>
-__global__ void __launch_bounds__(256) rfft_pre_kernel(const void* in0, const void* in1, float2* __restrict__ z,
-                                                       const float2* __restrict__ rtw, int logm, long long batch) {
+template <int LIN, class CX = float2>
+__global__ void __launch_bounds__(256) rfft_pre_kernel(const void* in0, const void* in1, CX* __restrict__ z,
+                                                       const CX* __restrict__ rtw, int logm, long long batch) {
+    using T = CxT<CX>;
+    using S = typename T::S;
     const long long m = 1ll << logm;
     const long long half = m >> 1;
     const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long f = g / (half + 1);
     const long long k = g - f * (half + 1);
     if (f >= batch) return;
-    float2* zf = z + f * m;
+    CX* zf = z + f * m;
     const long long fb = f * (m + 1);
     if (k == 0) {
-        const float r0 = ld_c(in0, in1, LIN, fb).x;
-        const float rm = ld_c(in0, in1, LIN, fb + m).x;
-        zf[0] = make_float2((r0 + rm) * 0.5f, (r0 - rm) * 0.5f);
+        const S r0 = ld_c<CX>(in0, in1, LIN, fb).x;
+        const S rm = ld_c<CX>(in0, in1, LIN, fb + m).x;
+        zf[0] = T::mk((r0 + rm) * S(0.5), (r0 - rm) * S(0.5));
         return;
     }
-    const float2 x1 = ld_c(in0, in1, LIN, fb + k);
-    const float2 x2 = ld_c(in0, in1, LIN, fb + (m - k));
-    const float2 w = rtw[k - 1];
-    const float si = x1.y + x2.y, dr = x1.x - x2.x;
+    const CX x1 = ld_c<CX>(in0, in1, LIN, fb + k);
+    const CX x2 = ld_c<CX>(in0, in1, LIN, fb + (m - k));
+    const CX w = rtw[k - 1];
+    const S si = x1.y + x2.y, dr = x1.x - x2.x;
     if (k != m - k) {
-        const float zr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(-si, w.x, x1.x));
-        const float zi_k = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x1.y));
-        zf[k] = make_float2(zr_k, zi_k);
+        const S zr_k = T::fma(-dr, w.y, T::fma(-si, w.x, x1.x));
+        const S zi_k = T::fma(dr, w.x, T::fma(-si, w.y, x1.y));
+        zf[k] = T::mk(zr_k, zi_k);
     }
-    const float zr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(si, w.x, x2.x));
-    const float zi_m = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x2.y));
-    zf[m - k] = make_float2(zr_m, zi_m);
+    const S zr_m = T::fma(dr, w.y, T::fma(si, w.x, x2.x));
+    const S zi_m = T::fma(dr, w.x, T::fma(-si, w.y, x2.y));
+    zf[m - k] = T::mk(zr_m, zi_m);
 }
 
 }  // namespace zl

@@ -19,3 +19,14 @@ void ZL_CAT(zl_fill_macro_, ZL_LOGL)(zl::MacroTable& tab) {
     tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA>();
     tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS>();
 }
+
+void ZL_CAT(zl_fill_macro64_, ZL_LOGL)(zl::MacroTable& tab) {
+    using namespace zl;
+    using GL = Geo<ZL_LOGL, 4, 1>;
+    constexpr int C = cfg_macro_c64(ZL_LOGL);
+    tab[0][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS, double2>();
+    tab[0][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS, double2>();
+    tab[1][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2>();
+    tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA, double2>();
+    tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS, double2>();
+}

@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFM
     const int tid = threadIdx.x;
     typename MfOp<P>::V ar[NS], ai[NS];
     dft16_operands<P, NS>(tid & 63, ar, ai);
-    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = a.tw[e];
+    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
     const long long f = blockIdx.x;
     float2 x[4][4];
     mfma4096_load(x, reinterpret_cast<const float2*>(a.in0) + f * 4096);
@@ -390,7 +390,7 @@ fft4096_mfma_stream_kernel(KArgs a) {
     const int tid = threadIdx.x;
     typename MfOp<P>::V ar[NS], ai[NS];
     dft16_operands<P, NS>(tid & 63, ar, ai);
-    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = a.tw[e];
+    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
     const float2* in = reinterpret_cast<const float2*>(a.in0);
     const float2* sink = reinterpret_cast<const float2*>(a.sink);
     long long f = blockIdx.x;

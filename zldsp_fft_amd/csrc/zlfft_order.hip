@@ -1,6 +1,7 @@
 // One CFFT order's LDS kernels (compiled once per order with -DZL_LOGN=n so
 // the build runs in parallel): C2C for the four AoS/SoA combinations and the
-// real kinds, whose inner CFFT has this order.
+// real kinds, whose inner CFFT has this order; FP32 and, up to
+// kMaxLdsOrder64, FP64.
 #include "zlfft_registry.hpp"
 
 #ifndef ZL_LOGN
@@ -22,4 +23,22 @@ void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
     tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS>();
     tab[C2R][AOS][AOS] = make_entry<G, C2R, AOS, AOS>();
     tab[C2R][SOA][AOS] = make_entry<G, C2R, SOA, AOS>();
+}
+
+void ZL_CAT(zl_fill_order64_, ZL_LOGN)(zl::OrderTable& tab) {
+    using namespace zl;
+#if ZL_LOGN <= 13
+    using G = Geo<ZL_LOGN, cfg_logr(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
+    tab[C2C][AOS][AOS] = make_entry64<G, C2C, AOS, AOS>();
+    tab[C2C][AOS][SOA] = make_entry64<G, C2C, AOS, SOA>();
+    tab[C2C][SOA][AOS] = make_entry64<G, C2C, SOA, AOS>();
+    tab[C2C][SOA][SOA] = make_entry64<G, C2C, SOA, SOA>();
+    tab[R2C][AOS][AOS] = make_entry64<G, R2C, AOS, AOS>();
+    tab[R2C][AOS][SOA] = make_entry64<G, R2C, AOS, SOA>();
+    tab[R2C_SQR][AOS][AOS] = make_entry64<G, R2C_SQR, AOS, AOS>();
+    tab[C2R][AOS][AOS] = make_entry64<G, C2R, AOS, AOS>();
+    tab[C2R][SOA][AOS] = make_entry64<G, C2R, SOA, AOS>();
+#else
+    (void)tab;  // FP64 runs this order as HBM passes
+#endif
 }

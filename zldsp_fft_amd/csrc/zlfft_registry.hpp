@@ -5,7 +5,8 @@
 
 namespace zl {
 
-constexpr int kMaxLdsOrder = 14;  // largest CFFT order kept whole in LDS
+constexpr int kMaxLdsOrder = 14;    // largest CFFT order kept whole in LDS (FP32)
+constexpr int kMaxLdsOrder64 = 13;  // the same for FP64 (2^14 double2 exceed the LDS)
 
 // Points-per-thread (log2) and target workgroup size of the LDS kernel for a
 // CFFT order.  Orders <= 4 keep a whole transform in one thread.
@@ -59,15 +60,36 @@ KernelEntry make_entry() {
     return e;
 }
 
+// FP64 entry: the LDS kernel over double2 (no streaming variant)
+template <class G, int KIND, int LIN, int LOUT>
+KernelEntry make_entry64() {
+    static_assert(8ll * 2 * G::TPW * G::LDSN <= 160 * 1024, "FP64 geometry exceeds the LDS");
+    KernelEntry e{};
+    e.fn = fft_lds_kernel<G, KIND, LIN, LOUT, double2>;
+    e.wg = G::WG;
+    e.tpw = G::TPW;
+    e.logn = G::logn;
+    e.np = G::NP;
+    for (int p = 0; p < G::NP && p < 8; ++p) {
+        e.lr[p] = G::lr(p);
+        e.lw[p] = G::lw(p);
+    }
+    e.tw_total = G::tw_total();
+    return e;
+}
+
 // ---- multi-pass (HBM) transforms for orders above kMaxLdsOrder
 
 constexpr int kMinMacroLogL = 6;
 constexpr int kMaxMacroLogL = 10;
-constexpr int kMaxOrder = 28;  // largest CFFT order
+constexpr int kMaxOrder = 28;    // largest CFFT order (FP32)
+constexpr int kMaxOrder64 = 27;  // largest CFFT order (FP64)
 constexpr int kRaggedMaxOrderTune = 14;  // orders of the ragged tuning table
 
-// columns (butterflies) per workgroup of a radix-2^logl HBM pass
+// columns (butterflies) per workgroup of a radix-2^logl HBM pass: FP32, FP64
+// (C columns of 2^logl complex values, padded, fill the LDS)
 constexpr int cfg_macro_c(int logl) { return logl <= 8 ? 32 : 16; }
+constexpr int cfg_macro_c64(int logl) { return logl <= 8 ? 32 : logl == 9 ? 16 : 8; }
 
 // radix (log2) of each HBM pass for a CFFT order above kMaxLdsOrder, first pass first
 struct MacroPlan {
@@ -94,6 +116,12 @@ constexpr MacroPlan macro_plan(int order) {
     }
 }
 
+// FP64: order 14 also runs as HBM passes
+constexpr MacroPlan macro_plan64(int order) {
+    if (order == 14) return {2, {7, 7}};
+    return order <= kMaxOrder64 ? macro_plan(order) : MacroPlan{0, {0, 0, 0, 0}};
+}
+
 struct MacroEntry {
     void (*fn)(MArgs);
     int wg;        // threads per workgroup
@@ -109,10 +137,11 @@ struct MacroEntry {
 // (layout = output), 2 = middle pass (layout unused, AoS)
 using MacroTable = MacroEntry[3][2];
 
-template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT>
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2>
 MacroEntry make_macro_entry() {
+    static_assert((long long)sizeof(CX) * C * GL::LDSN <= 160 * 1024, "HBM pass exceeds the LDS");
     MacroEntry e{};
-    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT>;
+    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT, CX>;
     e.wg = C * GL::T;
     e.cols = C;
     e.logl = GL::logn;
@@ -135,7 +164,9 @@ void* zl_mfma_kernel(int prec, int backward, int streaming);
 void* zl_ragged_kernel();
 void* zl_ragged_reset_kernel();
 
-#define ZL_DECLARE_MACRO(n) void zl_fill_macro_##n(zl::MacroTable& tab);
+#define ZL_DECLARE_MACRO(n)                            \
+    void zl_fill_macro_##n(zl::MacroTable& tab);       \
+    void zl_fill_macro64_##n(zl::MacroTable& tab);
 ZL_DECLARE_MACRO(6)
 ZL_DECLARE_MACRO(7)
 ZL_DECLARE_MACRO(8)
@@ -143,7 +174,9 @@ ZL_DECLARE_MACRO(9)
 ZL_DECLARE_MACRO(10)
 #undef ZL_DECLARE_MACRO
 
-#define ZL_DECLARE_ORDER(n) void zl_fill_order_##n(zl::OrderTable& tab);
+#define ZL_DECLARE_ORDER(n)                            \
+    void zl_fill_order_##n(zl::OrderTable& tab);       \
+    void zl_fill_order64_##n(zl::OrderTable& tab);
 ZL_DECLARE_ORDER(0)
 ZL_DECLARE_ORDER(1)
 ZL_DECLARE_ORDER(2)
