@@ -175,8 +175,9 @@ def _line(nbytes, flops, count, ms, rel=None):
 
 def config_lines(args) -> dict:
     """Throughput of the analyzer batch (4096 RFFTs of 2048 reals: AoS bins and
-    squared magnitudes) and of the large SoA CFFTs (orders 16/18/20/22 at
-    batch 64/16/4/1, forward and backward).  Each line cycles through
+    squared magnitudes), of CFFT<double> (8192 x 4096 points) and of the
+    large SoA CFFTs (orders 16/18/20/22 at batch 64/16/4/1, forward and
+    backward).  Each line cycles through
     ROTATE buffer sets; GB/s counts one read of the inputs and one write of
     the outputs; GFLOP/s uses 5 N log2 N per CFFT and 2.5 N log2 N per RFFT.
     rocFFT (through torch.fft) is timed beside it as an external reference
@@ -211,6 +212,20 @@ def config_lines(args) -> dict:
     out["rocfft_rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, ms)
     plan.close()
     del xs, ys, ps
+    torch.cuda.empty_cache()
+    # CFFT<double>: 8192 x 4096 points AoS (1 GiB of traffic per batch, like the headline batch)
+    n64, b64 = 4096, 8192
+    plan = zf.CFFT(12, zf.PREC_F64)
+    g = torch.Generator(device=dev).manual_seed(7)
+    ins = [torch.randn((b64, n64), dtype=torch.complex128, device=dev, generator=g) for _ in range(2)]
+    outs = [torch.empty_like(ins[0]) for _ in range(2)]
+    ms = _timed(torch, stream, [lambda i=i: plan.forward(ins[i], outs[i]) for i in range(2)], reps)
+    ref = oracle.cfft(ins[0][5].cpu().numpy(), True, "f64")
+    got = outs[0][5].cpu().numpy()
+    out["cfft64_o12_b8192_aos_forward"] = _line(32 * n64 * b64, 5.0 * n64 * 12 * b64, b64, ms,
+                                                 float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    plan.close()
+    del ins, outs
     torch.cuda.empty_cache()
     # large SoA CFFTs: 4,194,304 points per call
     for order, batch in ((16, 64), (18, 16), (20, 4), (22, 1)):
