@@ -171,6 +171,25 @@ int zlfft_ragged_create_config(zlfft_ragged *ragged, const zlfft_ragged_desc *de
 int zlfft_ragged_exec(zlfft_ragged ragged, void *stream);
 int zlfft_ragged_destroy(zlfft_ragged ragged);
 
+/* ---- STFT front-end ----
+ * No counterpart in the reference: it replaces the caller's framing and
+ * windowing loop in front of RFFT<F>::forward / forward_sqr_mag
+ * (reference src/zldsp_fft_rfft.hpp:68-79, 104-109), as an analyzer
+ * (spectrum display) would write it.  Frame f = 0..nframes-1 is
+ * signal[f * hop + n], n < N = 2^order (device memory, F values), multiplied
+ * by window[n] (device memory, N F values; NULL for none), and transformed:
+ *   ZLFFT_STFT_BINS : N/2+1 complex bins per frame in out_layout (as forward)
+ *   ZLFFT_STFT_POWER: N/2+1 squared magnitudes per frame (as forward_sqr_mag)
+ *   ZLFFT_STFT_DB   : 10 log10(max(|X|^2, db_floor)) per bin (db_floor > 0)
+ * LDS-resident orders run as one kernel that frames and windows at load time;
+ * larger orders gather the windowed frames into the plan workspace first.
+ * signal must hold (nframes - 1) * hop + N values (the call cannot check it). */
+#define ZLFFT_STFT_BINS 0
+#define ZLFFT_STFT_POWER 1
+#define ZLFFT_STFT_DB 2
+int zlfft_rfft_stft_exec(zlfft_plan plan, const void *signal, size_t hop, const void *window, size_t nframes,
+                         int output, int out_layout, void *out0, void *out1, float db_floor, void *stream);
+
 /* Copies the plan's RFFT post/pre twiddles (wc, ws) for k = 1..N/4 as 2*(N/4)
  * floats to host memory (reference src/common/zldsp_fft_common_init.hpp:347-388);
  * for tests that compare them with the FP64 oracle.  FP32 plans only

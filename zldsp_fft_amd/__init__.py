@@ -55,7 +55,9 @@ EXPORTS = (
     "zlfft_cfft_execute", "zlfft_rfft_forward", "zlfft_rfft_forward_sqr_mag", "zlfft_rfft_backward",
     "zlfft_plan_copy_rfft_twiddles", "zlfft_last_error", "zlfft_version",
     "zlfft_ragged_create", "zlfft_ragged_create_config", "zlfft_ragged_exec", "zlfft_ragged_destroy",
+    "zlfft_rfft_stft_exec",
 )
+STFT_BINS, STFT_POWER, STFT_DB = 0, 1, 2
 FORWARD_SQR_MAG = 2
 
 
@@ -112,6 +114,7 @@ def load_library() -> ctypes.CDLL:
                                              ctypes.POINTER(ctypes.c_int)]
     L.zlfft_ragged_exec.argtypes = [vp, vp]
     L.zlfft_ragged_destroy.argtypes = [vp]
+    L.zlfft_rfft_stft_exec.argtypes = [vp, vp, sz, vp, sz, ci, ci, vp, vp, ctypes.c_float, vp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if name not in ("zlfft_plan_get_size", "zlfft_plan_get_order", "zlfft_last_error"):
@@ -347,6 +350,45 @@ class RFFT(_Plan):
         if _nbytes(x) != self._ss * n or _nbytes(y) != self._ss * nb:
             raise ValueError(f"host buffers must hold {n} reals and {nb} floats")
         _check(self._lib.zlfft_rfft_forward_sqr_mag(self._h, _ptr(x), _ptr(y)))
+
+    def stft(self, signal, hop: int, out_buffer, window=None, output: int = STFT_BINS, db_floor: float = 1e-20,
+             nframes: int | None = None) -> int:
+        """STFT front-end (``zlfft_rfft_stft_exec``): frame f is
+        ``signal[f * hop : f * hop + N] * window`` (device tensors), transformed
+        to N/2+1 bins (AoS tensor or (real, imag) pair), squared magnitudes or
+        dB levels per frame, back to back in ``out_buffer``.  ``nframes``
+        defaults to every whole frame the signal holds.  Returns nframes."""
+        n = self.get_size()
+        nb = n // 2 + 1
+        x = _float_view(signal, "signal", self._f64)
+        if not _is_dev(x):
+            raise ValueError("signal must be a device tensor")
+        length = x.numel()
+        if hop < 0:
+            raise ValueError("hop must be >= 0")
+        if nframes is None:
+            if length < n:
+                raise ValueError("signal is shorter than one frame")
+            nframes = 1 if hop == 0 else (length - n) // hop + 1
+        if nframes > 0 and (nframes - 1) * hop + n > length:
+            raise ValueError(f"{nframes} frames at hop {hop} need {(nframes - 1) * hop + n} samples, have {length}")
+        w = None
+        if window is not None:
+            w = _float_view(window, "window", self._f64)
+            if w.numel() != n or not _is_dev(w):
+                raise ValueError(f"window must be a device tensor of {n} values")
+        if output == STFT_BINS:
+            so = _Side(out_buffer, "out_buffer", self._f64)
+            layout, o0, o1, have = so.layout, so.p0, so.p1, so.ncomplex
+            refs = so.ref
+        else:
+            y = _float_view(out_buffer, "out_buffer", self._f64)
+            layout, o0, o1, have, refs = AOS, _ptr(y), None, y.numel(), (y,)
+        if have != nframes * nb:
+            raise ValueError(f"out_buffer holds {have} values per side, need {nframes * nb}")
+        _check(self._lib.zlfft_rfft_stft_exec(self._h, _ptr(x), int(hop), None if w is None else _ptr(w), nframes,
+                                              int(output), layout, o0, o1, float(db_floor), _stream_of(x, *refs)))
+        return nframes
 
     def rfft_twiddles(self) -> np.ndarray:
         """The plan's (wc, ws) table for k = 1..N/4, copied to the host."""

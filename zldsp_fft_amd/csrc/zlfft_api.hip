@@ -313,7 +313,8 @@ size_t macro_work_bytes(zlfft_plan p, size_t batch) {
 
 // RFFT post-twiddle / squared magnitude and pre-twiddle around the multi-pass CFFT
 template <class CX>
-int rfft_post(zlfft_plan p, int kind, int lout, const void* z, void* out0, void* out1, size_t batch, void* stream) {
+int rfft_post(zlfft_plan p, int kind, int lout, const void* z, void* out0, void* out1, size_t batch, void* stream,
+              int db, float db_floor) {
     const size_t n = (size_t)1 << p->corder;
     const long long threads = (long long)batch * (long long)(n / 2 + 1);
     const unsigned grid = (unsigned)((threads + 255) / 256);
@@ -322,7 +323,7 @@ int rfft_post(zlfft_plan p, int kind, int lout, const void* z, void* out0, void*
     const CX* rtw = static_cast<const CX*>(p->d_rtw);
     if (kind == zl::R2C_SQR)
         zl::rfft_post_kernel<zl::R2C_SQR, zl::AOS, CX><<<grid, 256, 0, s>>>(zz, out0, nullptr, rtw, p->corder,
-                                                                           (long long)batch);
+                                                                           (long long)batch, db, db_floor);
     else if (lout == zl::AOS)
         zl::rfft_post_kernel<zl::R2C, zl::AOS, CX><<<grid, 256, 0, s>>>(zz, out0, out1, rtw, p->corder, (long long)batch);
     else
@@ -348,7 +349,7 @@ int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, s
 }
 
 int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
-                 size_t batch, int backward, void* stream) {
+                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f) {
     if (batch == 0) return ZLFFT_OK;
     if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
     void* work = p->d_work;
@@ -357,8 +358,8 @@ int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, con
     void* z = static_cast<char*>(work) + (p->mp > 2 ? 2 : 1) * batch * n * 2 * p->ss;
     if (kind == zl::R2C || kind == zl::R2C_SQR) {
         if (int rc = macro_cfft(p, 0, zl::AOS, in0, nullptr, zl::AOS, z, nullptr, batch, stream, work)) return rc;
-        return p->f64 ? rfft_post<double2>(p, kind, lout, z, out0, out1, batch, stream)
-                      : rfft_post<float2>(p, kind, lout, z, out0, out1, batch, stream);
+        return p->f64 ? rfft_post<double2>(p, kind, lout, z, out0, out1, batch, stream, db, db_floor)
+                      : rfft_post<float2>(p, kind, lout, z, out0, out1, batch, stream, db, db_floor);
     }
     // C2R
     if (int rc = p->f64 ? rfft_pre<double2>(p, lin, in0, in1, z, batch, stream)
@@ -774,6 +775,63 @@ int zlfft_rfft_backward(zlfft_plan p, int in_layout, const void* in0, const void
     return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void*, hipStream_t s) {
         return zlfft_rfft_backward_exec(p, in_layout, i0, i1, o0, 1, s);
     });
+}
+
+int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const void* window, size_t nframes, int output,
+                         int out_layout, void* out0, void* out1, float db_floor, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (output != ZLFFT_STFT_BINS && output != ZLFFT_STFT_POWER && output != ZLFFT_STFT_DB)
+        return fail(ZLFFT_ERR_INVALID_ARG, "output must be ZLFFT_STFT_BINS, _POWER or _DB");
+    if (output == ZLFFT_STFT_BINS && !valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    const int lout = output == ZLFFT_STFT_BINS ? out_layout : ZLFFT_AOS;
+    if (output == ZLFFT_STFT_DB && !(db_floor > 0.f)) return fail(ZLFFT_ERR_INVALID_ARG, "db_floor must be > 0");
+    if (hop > ((size_t)1 << 40)) return fail(ZLFFT_ERR_INVALID_ARG, "hop too large");
+    if (nframes && (!signal || !out0 || (lout == ZLFFT_SOA && !out1))) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    if (nframes == 0) return ZLFFT_OK;
+    const int db = output == ZLFFT_STFT_DB;
+    const int kind = output == ZLFFT_STFT_BINS ? zl::R2C : zl::R2C_SQR;
+    if (!p->macro) {
+        // one fused launch: framing and window at load time
+        const zl::KernelEntry& e = (*p->tab)[kind == zl::R2C ? zl::R2C_FRAMED : zl::R2C_SQR_FRAMED][zl::AOS][lout];
+        if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no framed kernel for this order");
+        const size_t groups = (nframes + e.tpw - 1) / e.tpw;
+        if (groups > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "too many frames for one launch");
+        zl::KArgs a{};
+        a.in0 = signal;
+        a.in1 = window;
+        a.out0 = out0;
+        a.out1 = out1;
+        a.tw = p->d_tw;
+        a.rtw = p->d_rtw;
+        a.batch = (long long)nframes;
+        a.hop = (long long)hop;
+        a.db = db;
+        a.db_floor = db_floor;
+        void* params[] = {&a};
+        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params, 0,
+                               reinterpret_cast<hipStream_t>(stream)));
+        return ZLFFT_OK;
+    }
+    // multi-pass sizes: the windowed frames go to the workspace (after the
+    // multi-pass buffers), then the RFFT runs on them
+    const size_t n = (size_t)1 << p->order;
+    const size_t mw = macro_work_bytes(p, nframes);
+    if (int rc = ensure_work(p, mw + nframes * n * p->ss)) return rc;
+    void* frames = static_cast<char*>(p->d_work) + mw;
+    const long long total = (long long)(nframes * n);
+    const unsigned grid = (unsigned)((total + 255) / 256);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (p->f64)
+        zl::frame_kernel<double><<<grid, 256, 0, s>>>(static_cast<const double*>(signal),
+                                                      static_cast<const double*>(window), static_cast<double*>(frames),
+                                                      (long long)hop, (long long)n, (long long)nframes);
+    else
+        zl::frame_kernel<float><<<grid, 256, 0, s>>>(static_cast<const float*>(signal), static_cast<const float*>(window),
+                                                     static_cast<float*>(frames), (long long)hop, (long long)n,
+                                                     (long long)nframes);
+    ZL_HIP(hipGetLastError());
+    return launch_macro(p, kind, 0, lout, frames, nullptr, out0, out1, nframes, 0, stream, db, db_floor);
 }
 
 int zlfft_plan_copy_rfft_twiddles(zlfft_plan p, float* host_out, size_t n_floats) {

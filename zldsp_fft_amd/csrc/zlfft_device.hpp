@@ -26,6 +26,8 @@
 namespace zl {
 
 enum Kind : int { C2C = 0, R2C = 1, R2C_SQR = 2, C2R = 3 };
+// kernel-table slots of the framed (STFT front-end) real-to-complex kernels
+enum FramedKind : int { R2C_FRAMED = 4, R2C_SQR_FRAMED = 5 };
 enum Layout : int { AOS = 0, SOA = 1 };
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -244,7 +246,39 @@ struct KArgs {
     int backward;      // C2C only
     float2* sink;      // kSinkBytes of scratch that inactive lanes of the
                        // streaming kernel read and write instead of user memory
+    // framed real input (STFT front-end, FRAMED kernels): frame f of the batch
+    // is signal[f * hop + n], n < N (in0 = signal), times window[n] (in1 =
+    // window, or null for none)
+    long long hop;
+    int db;            // R2C_SQR: write 10 log10(max(|X|^2, db_floor)) instead of |X|^2
+    float db_floor;
 };
+
+// Squared magnitude, or its level in dB when db is set (STFT front-end).
+template <class S>
+__device__ __forceinline__ S power_out(S p, int db, float db_floor) {
+    if (!db) return p;
+    const S q = p > (S)db_floor ? p : (S)db_floor;
+    if constexpr (sizeof(S) == 4)
+        return 10.0f * log10f(q);
+    else
+        return 10.0 * log10(q);
+}
+
+// Complex element m (real samples 2m, 2m + 1) of frame `frame` of a framed
+// real input, windowed.
+template <class CX>
+__device__ __forceinline__ CX framed_pair(const KArgs& a, long long frame, long long m) {
+    using S = typename CxT<CX>::S;
+    const S* x = static_cast<const S*>(a.in0) + frame * a.hop + 2 * m;
+    S x0 = x[0], x1 = x[1];
+    if (a.in1) {
+        const S* w = static_cast<const S*>(a.in1) + 2 * m;
+        x0 *= w[0];
+        x1 *= w[1];
+    }
+    return CxT<CX>::mk(x0, x1);
+}
 
 // size of KArgs::sink: two arrays of the largest LDS transform (2^14 complex)
 constexpr long long kSinkBytes = 2ll * 8 * (1 << 14);
@@ -275,8 +309,11 @@ __device__ __forceinline__ void st_c(void* p0, void* p1, int lay, long long e, C
 //   C2R    : M+1 bins in layout LIN -> N real
 //            (reference RFFT::backward, src/zldsp_fft_rfft.hpp:86-97, 132-138)
 // For the real kinds G describes the inner CFFT of size M = N/2.
-template <class G, int KIND, int LIN, int LOUT, class CX = float2>
+// FRAMED (R2C and R2C_SQR only): the real input is framed and windowed at
+// load time (STFT front-end, see KArgs::hop).
+template <class G, int KIND, int LIN, int LOUT, class CX = float2, int FRAMED = 0>
 __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
+    static_assert(!FRAMED || KIND == R2C || KIND == R2C_SQR, "framed input is real");
     using T = CxT<CX>;
     using S = typename T::S;
     __shared__ CX lds[G::TPW * G::LDSN];
@@ -329,8 +366,13 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
         if constexpr (direct_in) {
             const long long fb = f * M;
             if (active) {
+                if constexpr (FRAMED) {
 #pragma unroll
-                for (int j = 0; j < G::R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, fb + t + G::T * j);
+                    for (int j = 0; j < G::R; ++j) v[j] = framed_pair<CX>(a, f, t + G::T * j);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < G::R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, fb + t + G::T * j);
+                }
             } else {
 #pragma unroll
                 for (int j = 0; j < G::R; ++j) v[j] = T::mk(S(0), S(0));
@@ -343,7 +385,14 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
             for (int j = 0; j < G::R; ++j) {
                 const int e = tid + G::WG * j;
                 const int s = e / M, pos = e - s * M;
-                lds[s * G::LDSN + G::pad(pos)] = (e < wn) ? ld_c<CX>(a.in0, a.in1, lin, wb + e) : T::mk(S(0), S(0));
+                CX val = T::mk(S(0), S(0));
+                if (e < wn) {
+                    if constexpr (FRAMED)
+                        val = framed_pair<CX>(a, (long long)blockIdx.x * G::TPW + s, pos);
+                    else
+                        val = ld_c<CX>(a.in0, a.in1, lin, wb + e);
+                }
+                lds[s * G::LDSN + G::pad(pos)] = val;
             }
             __syncthreads();
 #pragma unroll
@@ -375,8 +424,8 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
                     st_c(a.out0, a.out1, LOUT, fb + M, T::mk(ny, S(0)));
                 } else {
                     S* o = reinterpret_cast<S*>(a.out0);
-                    o[fb] = dc * dc;
-                    o[fb + M] = ny * ny;
+                    o[fb] = power_out(dc * dc, a.db, a.db_floor);
+                    o[fb + M] = power_out(ny * ny, a.db, a.db_floor);
                 }
             }
             for (int k = 1 + t; k <= M / 2; k += G::T) {
@@ -393,8 +442,8 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
                     st_c(a.out0, a.out1, LOUT, fb + (M - k), T::mk(xr_m, xi_m));
                 } else {
                     S* o = reinterpret_cast<S*>(a.out0);
-                    if (k != M - k) o[fb + k] = T::fma(xr_k, xr_k, xi_k * xi_k);
-                    o[fb + (M - k)] = T::fma(xr_m, xr_m, xi_m * xi_m);
+                    if (k != M - k) o[fb + k] = power_out(T::fma(xr_k, xr_k, xi_k * xi_k), a.db, a.db_floor);
+                    o[fb + (M - k)] = power_out(T::fma(xr_m, xr_m, xi_m * xi_m), a.db, a.db_floor);
                 }
             }
         }
@@ -745,7 +794,8 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
 // (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552).
 template <int KIND, int LOUT, class CX = float2>
 __global__ void __launch_bounds__(256) rfft_post_kernel(const CX* __restrict__ z, void* out0, void* out1,
-                                                        const CX* __restrict__ rtw, int logm, long long batch) {
+                                                        const CX* __restrict__ rtw, int logm, long long batch,
+                                                        int db = 0, float db_floor = 0.f) {
     using T = CxT<CX>;
     using S = typename T::S;
     const long long m = 1ll << logm;
@@ -763,8 +813,8 @@ __global__ void __launch_bounds__(256) rfft_post_kernel(const CX* __restrict__ z
             st_c(out0, out1, LOUT, fb, T::mk(dc, S(0)));
             st_c(out0, out1, LOUT, fb + m, T::mk(ny, S(0)));
         } else {
-            reinterpret_cast<S*>(out0)[fb] = dc * dc;
-            reinterpret_cast<S*>(out0)[fb + m] = ny * ny;
+            reinterpret_cast<S*>(out0)[fb] = power_out(dc * dc, db, db_floor);
+            reinterpret_cast<S*>(out0)[fb + m] = power_out(ny * ny, db, db_floor);
         }
         return;
     }
@@ -780,9 +830,21 @@ __global__ void __launch_bounds__(256) rfft_post_kernel(const CX* __restrict__ z
         st_c(out0, out1, LOUT, fb + (m - k), T::mk(xr_m, xi_m));
     } else {
         S* o = reinterpret_cast<S*>(out0);
-        if (k != m - k) o[fb + k] = T::fma(xr_k, xr_k, xi_k * xi_k);
-        o[fb + (m - k)] = T::fma(xr_m, xr_m, xi_m * xi_m);
+        if (k != m - k) o[fb + k] = power_out(T::fma(xr_k, xr_k, xi_k * xi_k), db, db_floor);
+        o[fb + (m - k)] = power_out(T::fma(xr_m, xr_m, xi_m * xi_m), db, db_floor);
     }
+}
+
+// STFT front-end for the multi-pass sizes: frames (hop, window) of a signal
+// copied into a contiguous batch, out[f * n + i] = signal[f * hop + i] * window[i].
+template <class S>
+__global__ void __launch_bounds__(256) frame_kernel(const S* __restrict__ signal, const S* __restrict__ window,
+                                                    S* __restrict__ out, long long hop, long long n, long long frames) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= frames * n) return;
+    const long long f = g / n, i = g - f * n;
+    const S x = signal[f * hop + i];
+    out[g] = window ? x * window[i] : x;
 }
 
 // RFFT pre-twiddle for the multi-pass sizes: bins (layout LIN, M+1 per frame)

@@ -23,21 +23,27 @@ void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
     tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS>();
     tab[C2R][AOS][AOS] = make_entry<G, C2R, AOS, AOS>();
     tab[C2R][SOA][AOS] = make_entry<G, C2R, SOA, AOS>();
+    tab[R2C_FRAMED][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, float2, 1>();
+    tab[R2C_FRAMED][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA, float2, 1>();
+    tab[R2C_SQR_FRAMED][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS, float2, 1>();
 }
 
 void ZL_CAT(zl_fill_order64_, ZL_LOGN)(zl::OrderTable& tab) {
     using namespace zl;
 #if ZL_LOGN <= 13
     using G = Geo<ZL_LOGN, cfg_logr(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
-    tab[C2C][AOS][AOS] = make_entry64<G, C2C, AOS, AOS>();
-    tab[C2C][AOS][SOA] = make_entry64<G, C2C, AOS, SOA>();
-    tab[C2C][SOA][AOS] = make_entry64<G, C2C, SOA, AOS>();
-    tab[C2C][SOA][SOA] = make_entry64<G, C2C, SOA, SOA>();
-    tab[R2C][AOS][AOS] = make_entry64<G, R2C, AOS, AOS>();
-    tab[R2C][AOS][SOA] = make_entry64<G, R2C, AOS, SOA>();
-    tab[R2C_SQR][AOS][AOS] = make_entry64<G, R2C_SQR, AOS, AOS>();
-    tab[C2R][AOS][AOS] = make_entry64<G, C2R, AOS, AOS>();
-    tab[C2R][SOA][AOS] = make_entry64<G, C2R, SOA, AOS>();
+    tab[C2C][AOS][AOS] = make_lds_entry<G, C2C, AOS, AOS>();
+    tab[C2C][AOS][SOA] = make_lds_entry<G, C2C, AOS, SOA>();
+    tab[C2C][SOA][AOS] = make_lds_entry<G, C2C, SOA, AOS>();
+    tab[C2C][SOA][SOA] = make_lds_entry<G, C2C, SOA, SOA>();
+    tab[R2C][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS>();
+    tab[R2C][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA>();
+    tab[R2C_SQR][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS>();
+    tab[C2R][AOS][AOS] = make_lds_entry<G, C2R, AOS, AOS>();
+    tab[C2R][SOA][AOS] = make_lds_entry<G, C2R, SOA, AOS>();
+    tab[R2C_FRAMED][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, double2, 1>();
+    tab[R2C_FRAMED][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA, double2, 1>();
+    tab[R2C_SQR_FRAMED][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS, double2, 1>();
 #else
     (void)tab;  // FP64 runs this order as HBM passes
 #endif

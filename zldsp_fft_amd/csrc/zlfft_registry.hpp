@@ -28,8 +28,9 @@ struct KernelEntry {
     int img_hi[8]; // per pass: entries of the high factor part (0 if direct)
 };
 
-// table[kind][lin][lout] for one order; unused combinations stay null
-using OrderTable = KernelEntry[4][2][2];
+// table[kind][lin][lout] for one order (kinds C2C..C2R, then the framed
+// R2C_FRAMED / R2C_SQR_FRAMED); unused combinations stay null
+using OrderTable = KernelEntry[6][2][2];
 
 template <class G, int KIND, int LIN, int LOUT>
 KernelEntry make_entry() {
@@ -60,12 +61,13 @@ KernelEntry make_entry() {
     return e;
 }
 
-// FP64 entry: the LDS kernel over double2 (no streaming variant)
-template <class G, int KIND, int LIN, int LOUT>
-KernelEntry make_entry64() {
-    static_assert(8ll * 2 * G::TPW * G::LDSN <= 160 * 1024, "FP64 geometry exceeds the LDS");
+// LDS-kernel-only entry over complex type CX (FP64, and the framed STFT
+// kernels of both precisions): no streaming variant
+template <class G, int KIND, int LIN, int LOUT, class CX = double2, int FRAMED = 0>
+KernelEntry make_lds_entry() {
+    static_assert((long long)sizeof(CX) * G::TPW * G::LDSN <= 160 * 1024, "geometry exceeds the LDS");
     KernelEntry e{};
-    e.fn = fft_lds_kernel<G, KIND, LIN, LOUT, double2>;
+    e.fn = fft_lds_kernel<G, KIND, LIN, LOUT, CX, FRAMED>;
     e.wg = G::WG;
     e.tpw = G::TPW;
     e.logn = G::logn;
