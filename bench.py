@@ -210,6 +210,17 @@ def config_lines(args) -> dict:
                                            float(np.linalg.norm(got - np.abs(ref) ** 2) / np.linalg.norm(np.abs(ref) ** 2)))
     ms = _timed(torch, stream, [lambda i=i: torch.fft.rfft(xs[i], dim=-1) for i in range(ROTATE)], reps)
     out["rocfft_rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, ms)
+    # STFT front-end: 4096 Hann-windowed frames of 2048 samples at hop 512 of
+    # one signal, dB levels (frames cut and windowed at load time); bytes count
+    # the frame samples read and the levels written
+    hop = 512
+    sig = [torch.rand((frames - 1) * hop + n, device=dev) * 2 - 1 for _ in range(ROTATE)]
+    win = torch.from_numpy(signals.hann(n).astype(np.float32)).to(dev)
+    lv = [torch.empty(frames * nb, dtype=torch.float32, device=dev) for _ in range(ROTATE)]
+    ms = _timed(torch, stream, [lambda i=i: plan.stft(sig[i], hop, lv[i], window=win, output=zf.STFT_DB,
+                                                       db_floor=1e-12, nframes=frames) for i in range(ROTATE)], reps)
+    out["stft2048_hop512_b4096_hann_db"] = _line(frames * (4 * n + 4 * nb), flops, frames, ms)
+    del sig, lv
     plan.close()
     del xs, ys, ps
     torch.cuda.empty_cache()
