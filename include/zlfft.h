@@ -48,6 +48,12 @@
 extern "C" {
 #endif
 
+/* The library is built with hidden symbol visibility; these declarations are
+ * its exported interface. */
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility push(default)
+#endif
+
 #define ZLFFT_OK 0
 #define ZLFFT_ERR_INVALID_ORDER 1
 #define ZLFFT_ERR_NULL 2
@@ -201,6 +207,10 @@ const char *zlfft_last_error(void);
 
 /* Library version as MAJOR * 10000 + MINOR * 100 + PATCH. */
 int zlfft_version(void);
+
+#if defined(__GNUC__) || defined(__clang__)
+#pragma GCC visibility pop
+#endif
 
 #ifdef __cplusplus
 }

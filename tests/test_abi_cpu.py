@@ -27,6 +27,15 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(zf.EXPORTS)
 
 
+def test_library_exports_only_the_c_abi():
+    """Built with hidden visibility: the dynamic symbol table holds exactly the
+    zlfft_* functions that include/zlfft.h declares."""
+    import subprocess
+    out = subprocess.run(["nm", "-D", "--defined-only", zf.lib_path], capture_output=True, text=True, check=True).stdout
+    syms = {line.split()[-1] for line in out.splitlines() if len(line.split()) >= 3}
+    assert syms == set(_declared()), sorted(syms ^ set(_declared()))[:10]
+
+
 def test_version():
     assert zf.load_library().zlfft_version() >= 100
 

@@ -27,6 +27,7 @@ HIP_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    "-fvisibility=hidden",  # only the zlfft_* C-ABI (include/zlfft.h) is exported
     "-ffp-contract=off",
     # packed FP32 (v_pk_*_f32) is kept off for every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -64,8 +65,13 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
             if rc:
                 raise RuntimeError(f"hipcc failed ({rc}) for {cmd[-3]}")
     objs = [obj for _, obj, _ in units]
+    # export the C-ABI only: the kernels' host-side handles stay local too
+    vmap = os.path.join(OBJ, "exports.map")
+    with open(vmap, "w") as f:
+        f.write("{ global: zlfft_*; local: *; };\n")
     tmp = LIB + ".tmp"
-    cmd, rc, err = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    cmd, rc, err = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", f"-Wl,--version-script={vmap}", "-o", tmp,
+                         *objs])
     if rc:
         print(err, file=sys.stderr)
         raise RuntimeError("link of libzlfft.so failed")
