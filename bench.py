@@ -238,6 +238,24 @@ def config_lines(args) -> dict:
     plan.close()
     del ins, outs
     torch.cuda.empty_cache()
+    # host buffers (PCIe-inclusive): the headline batch from host memory through
+    # the pipelined host path, pageable and pinned, wall-clock per call
+    import time as _time
+    xh = np.random.default_rng(1).standard_normal((BATCH, 2 * N)).astype(np.float32).view(np.complex64)
+    yh = np.empty_like(xh)
+    plan = zf.CFFT(ORDER)
+    for name, (src, dst) in (("pageable", (xh, yh)),
+                             ("pinned", (torch.from_numpy(xh).pin_memory(), torch.empty(xh.shape, dtype=torch.complex64).pin_memory()))):
+        plan.forward(src, dst)  # warm-up: staging allocation
+        t = []
+        for _ in range(3):
+            t0 = _time.perf_counter()
+            plan.forward(src, dst)
+            t.append(_time.perf_counter() - t0)
+        ms = 1e3 * float(np.median(t))
+        out[f"host_{name}_cfft4096_b16384"] = _line(BYTES_PER_STEP, 5.0 * N * ORDER * BATCH, BATCH, ms)
+    plan.close()
+    del xh, yh
     # large SoA CFFTs: 4,194,304 points per call
     for order, batch in ((16, 64), (18, 16), (20, 4), (22, 1)):
         n = 1 << order

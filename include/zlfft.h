@@ -144,6 +144,21 @@ int zlfft_rfft_forward(zlfft_plan plan, const void *in, int out_layout, void *ou
 int zlfft_rfft_forward_sqr_mag(zlfft_plan plan, const void *in, void *out);
 int zlfft_rfft_backward(zlfft_plan plan, int in_layout, const void *in0, const void *in1, void *out);
 
+/* Batched forms of the synchronous calls: `batch` transforms back to back in
+ * host or device memory, returning when every output is in place.  Host
+ * batches run as a pipeline of chunks through pinned staging on two streams,
+ * so host-to-device copies, kernels and device-to-host copies overlap; pinned
+ * host buffers (hipHostMalloc) are transferred directly, pageable ones are
+ * copied through the staging.  No counterpart in the reference (its calls
+ * take one transform); a loop of reference calls over a batch becomes one
+ * call. */
+int zlfft_cfft_execute_batch(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
+                             int out_layout, void *out0, void *out1, size_t batch);
+int zlfft_rfft_forward_batch(zlfft_plan plan, const void *in, int out_layout, void *out0, void *out1, size_t batch);
+int zlfft_rfft_forward_sqr_mag_batch(zlfft_plan plan, const void *in, void *out, size_t batch);
+int zlfft_rfft_backward_batch(zlfft_plan plan, int in_layout, const void *in0, const void *in1, void *out,
+                              size_t batch);
+
 /* ---- ragged batches ----
  * Many transforms of different orders, kinds, directions and layouts run as
  * one persistent kernel, captured once in a hipGraph: zlfft_ragged_exec is a

@@ -12,10 +12,12 @@ C-ABI in ``include/zlfft.h`` (``libzlfft.so``, built in-tree by
 * ``RFFT(order)``: ``forward(real, out)``, ``backward(in, real)``,
   ``forward_sqr_mag(real, out)``, ``get_size()``, ``get_order()``.
 
-Host buffers (numpy arrays or CPU tensors) hold one transform and the call is
-synchronous, like the reference.  CUDA (HIP) tensors may hold a batch of
-transforms back to back (``numel`` a multiple of the per-transform size); the
-whole batch runs as one launch on torch's current stream.
+Host buffers (numpy arrays or CPU tensors, pinned or not) may hold one
+transform or a batch back to back, and the call is synchronous, like the
+reference's; batches run through the library's pipelined host path.  CUDA
+(HIP) tensors may hold a batch of transforms back to back (``numel`` a
+multiple of the per-transform size); the whole batch runs as one launch on
+torch's current stream.
 
 There is no CPU fallback: if ``libzlfft.so`` is missing or no GPU is present
 the calls raise.
@@ -56,6 +58,8 @@ EXPORTS = (
     "zlfft_plan_copy_rfft_twiddles", "zlfft_last_error", "zlfft_version",
     "zlfft_ragged_create", "zlfft_ragged_create_config", "zlfft_ragged_exec", "zlfft_ragged_destroy",
     "zlfft_rfft_stft_exec",
+    "zlfft_cfft_execute_batch", "zlfft_rfft_forward_batch", "zlfft_rfft_forward_sqr_mag_batch",
+    "zlfft_rfft_backward_batch",
 )
 STFT_BINS, STFT_POWER, STFT_DB = 0, 1, 2
 FORWARD_SQR_MAG = 2
@@ -114,6 +118,10 @@ def load_library() -> ctypes.CDLL:
                                              ctypes.POINTER(ctypes.c_int)]
     L.zlfft_ragged_exec.argtypes = [vp, vp]
     L.zlfft_ragged_destroy.argtypes = [vp]
+    L.zlfft_cfft_execute_batch.argtypes = [vp, ci, ci, vp, vp, ci, vp, vp, sz]
+    L.zlfft_rfft_forward_batch.argtypes = [vp, vp, ci, vp, vp, sz]
+    L.zlfft_rfft_forward_sqr_mag_batch.argtypes = [vp, vp, vp, sz]
+    L.zlfft_rfft_backward_batch.argtypes = [vp, ci, vp, vp, vp, sz]
     L.zlfft_rfft_stft_exec.argtypes = [vp, vp, sz, vp, sz, ci, ci, vp, vp, ctypes.c_float, vp]
     for name in EXPORTS:
         fn = getattr(L, name)
@@ -281,9 +289,11 @@ class CFFT(_Plan):
             _check(self._lib.zlfft_cfft_exec(self._h, direction, si.layout, si.p0, si.p1, so.layout, so.p0, so.p1,
                                              b, _stream_of(*si.ref, *so.ref)))
             return
-        if si.ncomplex != n or so.ncomplex != n:
-            raise ValueError(f"host buffers must hold exactly {n} complex values")
-        _check(self._lib.zlfft_cfft_execute(self._h, direction, si.layout, si.p0, si.p1, so.layout, so.p0, so.p1))
+        b = _batch(si.ncomplex, n, "in_buffer")
+        if _batch(so.ncomplex, n, "out_buffer") != b:
+            raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _check(self._lib.zlfft_cfft_execute_batch(self._h, direction, si.layout, si.p0, si.p1, so.layout, so.p0, so.p1,
+                                                  b))
 
     def forward(self, in_buffer, out_buffer) -> None:
         """AoS/SoA -> AoS/SoA forward (reference ``src/zldsp_fft_cfft.hpp:38-95``)."""
@@ -314,9 +324,10 @@ class RFFT(_Plan):
             _check(self._lib.zlfft_rfft_forward_exec(self._h, _ptr(x), so.layout, so.p0, so.p1, b,
                                                      _stream_of(x, *so.ref)))
             return
-        if _nbytes(x) != self._ss * n or so.ncomplex != nb:
-            raise ValueError(f"host buffers must hold {n} reals and {nb} bins")
-        _check(self._lib.zlfft_rfft_forward(self._h, _ptr(x), so.layout, so.p0, so.p1))
+        b = _batch(_nbytes(x) // self._ss, n, "in_buffer")
+        if _batch(so.ncomplex, nb, "out_buffer") != b:
+            raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _check(self._lib.zlfft_rfft_forward_batch(self._h, _ptr(x), so.layout, so.p0, so.p1, b))
 
     def backward(self, in_buffer, out_buffer) -> None:
         """N/2+1 bins (AoS or SoA) -> N reals (reference ``src/zldsp_fft_rfft.hpp:86-97``)."""
@@ -331,9 +342,10 @@ class RFFT(_Plan):
             _check(self._lib.zlfft_rfft_backward_exec(self._h, si.layout, si.p0, si.p1, _ptr(y), b,
                                                       _stream_of(*si.ref, y)))
             return
-        if si.ncomplex != nb or _nbytes(y) != self._ss * n:
-            raise ValueError(f"host buffers must hold {nb} bins and {n} reals")
-        _check(self._lib.zlfft_rfft_backward(self._h, si.layout, si.p0, si.p1, _ptr(y)))
+        b = _batch(si.ncomplex, nb, "in_buffer")
+        if _batch(_nbytes(y) // self._ss, n, "out_buffer") != b:
+            raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _check(self._lib.zlfft_rfft_backward_batch(self._h, si.layout, si.p0, si.p1, _ptr(y), b))
 
     def forward_sqr_mag(self, in_buffer, out_buffer) -> None:
         """N reals -> N/2+1 squared magnitudes (reference ``src/zldsp_fft_rfft.hpp:104-109``)."""
@@ -347,9 +359,10 @@ class RFFT(_Plan):
                 raise ValueError("in_buffer and out_buffer hold different batch sizes")
             _check(self._lib.zlfft_rfft_forward_sqr_mag_exec(self._h, _ptr(x), _ptr(y), b, _stream_of(x, y)))
             return
-        if _nbytes(x) != self._ss * n or _nbytes(y) != self._ss * nb:
-            raise ValueError(f"host buffers must hold {n} reals and {nb} floats")
-        _check(self._lib.zlfft_rfft_forward_sqr_mag(self._h, _ptr(x), _ptr(y)))
+        b = _batch(_nbytes(x) // self._ss, n, "in_buffer")
+        if _batch(_nbytes(y) // self._ss, nb, "out_buffer") != b:
+            raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _check(self._lib.zlfft_rfft_forward_sqr_mag_batch(self._h, _ptr(x), _ptr(y), b))
 
     def stft(self, signal, hop: int, out_buffer, window=None, output: int = STFT_BINS, db_floor: float = 1e-20,
              nframes: int | None = None) -> int:

@@ -5,11 +5,14 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/zlfft.h"
 #include "zlfft_registry.hpp"
@@ -235,9 +238,16 @@ struct zlfft_plan_s {
     void* d_twb[4] = {};
     void* d_work = nullptr;     // intermediate buffers, grown on demand
     size_t work_bytes = 0;
-    // synchronous calls on host pointers: staging buffers and a stream
-    void* d_stage = nullptr;
-    size_t stage_bytes = 0;
+    // calls on host pointers: a pipeline of slots (see run_host)
+    struct HostSlot {
+        void* pin = nullptr;        // pinned host staging of one chunk
+        void* dev = nullptr;        // device staging of one chunk
+        size_t bytes = 0;
+        hipStream_t stream = nullptr;      // copy-in and kernels
+        hipStream_t stream_out = nullptr;  // copy-out, after `computed`
+        hipEvent_t computed = nullptr;
+        hipEvent_t done = nullptr;
+    } slot[4];
     hipStream_t stream = nullptr;
 };
 
@@ -427,64 +437,201 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
 
 bool valid_layout(int l) { return l == ZLFFT_AOS || l == ZLFFT_SOA; }
 
-// true when ptr is device (or managed) memory the GPU can address directly
-bool is_device_ptr(const void* ptr) {
+// Where a user buffer lives, for the calls that accept host pointers.
+enum MemKind { MEM_PAGEABLE = 0, MEM_PINNED = 1, MEM_DEVICE = 2 };
+
+MemKind mem_kind(const void* ptr) {
     hipPointerAttribute_t attr;
     std::memset(&attr, 0, sizeof(attr));
     hipError_t e = hipPointerGetAttributes(&attr, ptr);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        return false;
+        return MEM_PAGEABLE;
     }
-    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+    if (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged) return MEM_DEVICE;
+    if (attr.type == hipMemoryTypeHost) return MEM_PINNED;
+    return MEM_PAGEABLE;
 }
 
-int ensure_stage(zlfft_plan p, size_t bytes) {
-    if (p->stage_bytes >= bytes) return ZLFFT_OK;
-    if (p->d_stage) {
-        (void)hipFree(p->d_stage);
-        p->d_stage = nullptr;
-        p->stage_bytes = 0;
+// One side (input or output) of a call on host or device buffers: up to two
+// arrays, `bytes` per transform each.
+struct Buf {
+    const void* p[2];
+    size_t bytes[2];
+};
+
+constexpr size_t kAlign = 256;
+constexpr int kMaxSlots = 4;
+
+// Pipeline shape: bytes of inputs + outputs per chunk and slots in turn
+// (ZLFFT_HOST_CHUNK_MB and ZLFFT_HOST_SLOTS override the defaults).
+struct HostShape {
+    size_t chunk_bytes = 32u << 20;
+    int slots = 3;
+};
+
+// memcpy of a large pageable block on several host threads (the single-thread
+// copy is slower than the link)
+void par_memcpy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kMinPerThread = 4u << 20;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = std::min<size_t>({8, hw, std::max<size_t>(1, bytes / kMinPerThread)});
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
     }
-    hipError_t e = hipMalloc(&p->d_stage, bytes);
-    if (e != hipSuccess) return fail(ZLFFT_ERR_ALLOC, std::string("staging hipMalloc: ") + hipGetErrorString(e));
-    p->stage_bytes = bytes;
+    const size_t part = (bytes + nt - 1) / nt;
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (size_t i = 1; i < nt; ++i) {
+        const size_t o = i * part;
+        if (o >= bytes) break;
+        th.emplace_back([=] { std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o,
+                                          std::min(part, bytes - o)); });
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (auto& t : th) t.join();
+}
+HostShape host_shape() {
+    static const HostShape s = [] {
+        HostShape h;
+        if (const char* e = getenv("ZLFFT_HOST_CHUNK_MB")) {
+            const long v = atol(e);
+            if (v >= 1 && v <= 1024) h.chunk_bytes = (size_t)v << 20;
+        }
+        if (const char* e = getenv("ZLFFT_HOST_SLOTS")) {
+            const int v = atoi(e);
+            if (v >= 1 && v <= kMaxSlots) h.slots = v;
+        }
+        return h;
+    }();
+    return s;
+}
+
+size_t round_up(size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int ensure_slot(zlfft_plan p, int i, size_t bytes) {
+    auto& sl = p->slot[i];
+    if (!sl.stream) {
+        ZL_HIP(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        ZL_HIP(hipStreamCreateWithFlags(&sl.stream_out, hipStreamNonBlocking));
+        ZL_HIP(hipEventCreateWithFlags(&sl.computed, hipEventDisableTiming));
+        ZL_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    }
+    if (sl.bytes >= bytes) return ZLFFT_OK;
+    ZL_HIP(hipStreamSynchronize(sl.stream));
+    ZL_HIP(hipStreamSynchronize(sl.stream_out));
+    if (sl.pin) (void)hipHostFree(sl.pin);
+    if (sl.dev) (void)hipFree(sl.dev);
+    sl.pin = sl.dev = nullptr;
+    sl.bytes = 0;
+    if (hipHostMalloc(&sl.pin, bytes, hipHostMallocDefault) != hipSuccess || hipMalloc(&sl.dev, bytes) != hipSuccess)
+        return fail(ZLFFT_ERR_ALLOC, "host pipeline staging allocation failed");
+    sl.bytes = bytes;
     return ZLFFT_OK;
 }
 
-// Host/device buffer description for the synchronous calls
-struct Buf {
-    const void* p0;
-    const void* p1;
-    size_t bytes0, bytes1;  // bytes of each array
-};
-
-// Runs `body(in0, in1, out0, out1)` on device pointers, staging host buffers.
+// Runs body(in0, in1, out0, out1, count, stream) over `batch` transforms whose
+// buffers may be pageable host, pinned host or device memory, and returns when
+// the outputs are in place.  All-device calls run directly on the plan stream.
+// Otherwise the batch goes through in chunks (32 MiB of inputs + outputs by
+// default), taking the slots in turn (three by default; each is pinned
+// staging + device staging + a stream for copy-in and kernels + a stream for
+// copy-out + events): the copy-in of
+// one chunk, the kernels of the previous one and the copy-out of the one
+// before overlap (the copy engines and the GPU run concurrently).  Pageable
+// arrays are copied through the pinned staging by the host; pinned arrays are
+// transferred directly; device arrays are used in place.  Multi-pass plans
+// share one workspace, so they use one slot.
 template <class Body>
-int run_sync(zlfft_plan p, Buf in, Buf out, Body body) {
-    const bool dev = is_device_ptr(in.p0) && (!in.p1 || is_device_ptr(in.p1)) && is_device_ptr(out.p0) &&
-                     (!out.p1 || is_device_ptr(out.p1));
-    if (dev) {
-        int rc = body(in.p0, in.p1, const_cast<void*>(out.p0), const_cast<void*>(out.p1), p->stream);
+int run_host(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
+    MemKind kin[2] = {MEM_DEVICE, MEM_DEVICE}, kout[2] = {MEM_DEVICE, MEM_DEVICE};
+    bool all_dev = true;
+    for (int k = 0; k < 2; ++k) {
+        if (in.p[k]) all_dev &= (kin[k] = mem_kind(in.p[k])) == MEM_DEVICE;
+        if (out.p[k]) all_dev &= (kout[k] = mem_kind(out.p[k])) == MEM_DEVICE;
+    }
+    if (all_dev) {
+        int rc = body(in.p[0], in.p[1], const_cast<void*>(out.p[0]), const_cast<void*>(out.p[1]), batch, p->stream);
         if (rc) return rc;
         ZL_HIP(hipStreamSynchronize(p->stream));
         return ZLFFT_OK;
     }
-    const size_t total = in.bytes0 + in.bytes1 + out.bytes0 + out.bytes1;
-    int rc = ensure_stage(p, total + 256);
-    if (rc) return rc;
-    char* base = static_cast<char*>(p->d_stage);
-    char* di0 = base;
-    char* di1 = di0 + in.bytes0;
-    char* do0 = di1 + in.bytes1;
-    char* do1 = do0 + out.bytes0;
-    ZL_HIP(hipMemcpyAsync(di0, in.p0, in.bytes0, hipMemcpyDefault, p->stream));
-    if (in.p1) ZL_HIP(hipMemcpyAsync(di1, in.p1, in.bytes1, hipMemcpyDefault, p->stream));
-    rc = body(di0, in.p1 ? di1 : nullptr, do0, out.p1 ? do1 : nullptr, p->stream);
-    if (rc) return rc;
-    ZL_HIP(hipMemcpyAsync(const_cast<void*>(out.p0), do0, out.bytes0, hipMemcpyDefault, p->stream));
-    if (out.p1) ZL_HIP(hipMemcpyAsync(const_cast<void*>(out.p1), do1, out.bytes1, hipMemcpyDefault, p->stream));
-    ZL_HIP(hipStreamSynchronize(p->stream));
+    const size_t per = in.bytes[0] + in.bytes[1] + out.bytes[0] + out.bytes[1];
+    const HostShape shape = host_shape();
+    const size_t chunk = std::max<size_t>(1, std::min<size_t>(batch, shape.chunk_bytes / std::max<size_t>(per, 1)));
+    // slot layout: in0 | in1 | out0 | out1, each kAlign-aligned
+    size_t off[4], o = 0;
+    const size_t bsz[4] = {in.bytes[0], in.bytes[1], out.bytes[0], out.bytes[1]};
+    for (int k = 0; k < 4; ++k) {
+        off[k] = o;
+        o += round_up(chunk * bsz[k]);
+    }
+    const int nslots = p->macro ? 1 : shape.slots;
+    for (int i = 0; i < nslots; ++i)
+        if (int rc = ensure_slot(p, i, o)) return rc;
+    size_t pend_first[kMaxSlots] = {}, pend_n[kMaxSlots] = {};
+    // waits for a slot's chunk and moves its outputs from pinned staging to pageable user memory
+    auto finish = [&](int i) -> int {
+        if (!pend_n[i]) return ZLFFT_OK;
+        auto& sl = p->slot[i];
+        ZL_HIP(hipEventSynchronize(sl.done));
+        for (int k = 0; k < 2; ++k)
+            if (out.p[k] && kout[k] == MEM_PAGEABLE)
+                par_memcpy(static_cast<char*>(const_cast<void*>(out.p[k])) + pend_first[i] * out.bytes[k],
+                           static_cast<char*>(sl.pin) + off[2 + k], pend_n[i] * out.bytes[k]);
+        pend_n[i] = 0;
+        return ZLFFT_OK;
+    };
+    int c = 0;
+    for (size_t first = 0; first < batch; first += chunk, ++c) {
+        const size_t n = std::min(chunk, batch - first);
+        const int i = c % nslots;
+        if (int rc = finish(i)) return rc;
+        auto& sl = p->slot[i];
+        char* pin = static_cast<char*>(sl.pin);
+        char* dev = static_cast<char*>(sl.dev);
+        const void* din[2] = {nullptr, nullptr};
+        void* dout[2] = {nullptr, nullptr};
+        for (int k = 0; k < 2; ++k) {
+            if (!in.p[k]) continue;
+            const char* src = static_cast<const char*>(in.p[k]) + first * in.bytes[k];
+            const size_t nb = n * in.bytes[k];
+            if (kin[k] == MEM_DEVICE) {
+                din[k] = src;
+            } else {
+                const void* from = src;
+                if (kin[k] == MEM_PAGEABLE) {
+                    par_memcpy(pin + off[k], src, nb);
+                    from = pin + off[k];
+                }
+                ZL_HIP(hipMemcpyAsync(dev + off[k], from, nb, hipMemcpyHostToDevice, sl.stream));
+                din[k] = dev + off[k];
+            }
+        }
+        for (int k = 0; k < 2; ++k) {
+            if (!out.p[k]) continue;
+            dout[k] = kout[k] == MEM_DEVICE ? static_cast<char*>(const_cast<void*>(out.p[k])) + first * out.bytes[k]
+                                            : static_cast<void*>(dev + off[2 + k]);
+        }
+        if (int rc = body(din[0], din[1], dout[0], dout[1], n, sl.stream)) return rc;
+        // copy-out on a stream of its own, so the two copy directions of
+        // neighbouring chunks run at the same time
+        ZL_HIP(hipEventRecord(sl.computed, sl.stream));
+        ZL_HIP(hipStreamWaitEvent(sl.stream_out, sl.computed, 0));
+        for (int k = 0; k < 2; ++k) {
+            if (!out.p[k] || kout[k] == MEM_DEVICE) continue;
+            void* to = kout[k] == MEM_PINNED ? static_cast<char*>(const_cast<void*>(out.p[k])) + first * out.bytes[k]
+                                             : static_cast<void*>(pin + off[2 + k]);
+            ZL_HIP(hipMemcpyAsync(to, dev + off[2 + k], n * out.bytes[k], hipMemcpyDeviceToHost, sl.stream_out));
+        }
+        ZL_HIP(hipEventRecord(sl.done, sl.stream_out));
+        pend_first[i] = first;
+        pend_n[i] = n;
+    }
+    // the older chunk first
+    for (int j = 0; j < nslots; ++j)
+        if (int rc = finish((c + j) % nslots)) return rc;
     return ZLFFT_OK;
 }
 
@@ -666,7 +813,16 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (p->d_sink) (void)hipFree(p->d_sink);
     if (p->d_tw16) (void)hipFree(p->d_tw16);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
-    if (p->d_stage) (void)hipFree(p->d_stage);
+    for (auto& sl : p->slot) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.stream_out) (void)hipStreamSynchronize(sl.stream_out);
+        if (sl.pin) (void)hipHostFree(sl.pin);
+        if (sl.dev) (void)hipFree(sl.dev);
+        if (sl.computed) (void)hipEventDestroy(sl.computed);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.stream_out) (void)hipStreamDestroy(sl.stream_out);
+    }
     if (p->d_work) (void)hipFree(p->d_work);
     for (int q = 0; q < 4; ++q) {
         if (p->d_itw[q]) (void)hipFree(p->d_itw[q]);
@@ -720,61 +876,81 @@ int zlfft_rfft_backward_exec(zlfft_plan p, int in_layout, const void* in0, const
     return launch(p, zl::C2R, in_layout, 0, in0, in1, out, nullptr, batch, 1, stream);
 }
 
-int zlfft_cfft_execute(zlfft_plan p, int direction, int in_layout, const void* in0, const void* in1, int out_layout,
-                       void* out0, void* out1) {
+int zlfft_cfft_execute_batch(zlfft_plan p, int direction, int in_layout, const void* in0, const void* in1,
+                             int out_layout, void* out0, void* out1, size_t batch) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_CFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not a CFFT plan");
     if (!valid_layout(in_layout) || !valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch == 0) return ZLFFT_OK;
     if (!in0 || !out0 || (in_layout == ZLFFT_SOA && !in1) || (out_layout == ZLFFT_SOA && !out1))
         return fail(ZLFFT_ERR_NULL, "NULL data pointer");
     const size_t n = (size_t)1 << p->order, ss = p->ss;
-    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 2 * ss * n : ss * n,
-                 in_layout == ZLFFT_SOA ? ss * n : 0};
-    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 2 * ss * n : ss * n,
-                 out_layout == ZLFFT_SOA ? ss * n : 0};
-    return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void* o1, hipStream_t s) {
-        return zlfft_cfft_exec(p, direction, in_layout, i0, i1, out_layout, o0, o1, 1, s);
+    const bool si = in_layout == ZLFFT_SOA, so = out_layout == ZLFFT_SOA;
+    const Buf bi{{in0, si ? in1 : nullptr}, {si ? ss * n : 2 * ss * n, si ? ss * n : 0}};
+    const Buf bo{{out0, so ? out1 : nullptr}, {so ? ss * n : 2 * ss * n, so ? ss * n : 0}};
+    return run_host(p, bi, bo, batch, [&](const void* i0, const void* i1, void* o0, void* o1, size_t cnt, hipStream_t s) {
+        return zlfft_cfft_exec(p, direction, in_layout, i0, i1, out_layout, o0, o1, cnt, s);
+    });
+}
+
+int zlfft_cfft_execute(zlfft_plan p, int direction, int in_layout, const void* in0, const void* in1, int out_layout,
+                       void* out0, void* out1) {
+    return zlfft_cfft_execute_batch(p, direction, in_layout, in0, in1, out_layout, out0, out1, 1);
+}
+
+int zlfft_rfft_forward_batch(zlfft_plan p, const void* in, int out_layout, void* out0, void* out1, size_t batch) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
+    if (!valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch == 0) return ZLFFT_OK;
+    if (!in || !out0 || (out_layout == ZLFFT_SOA && !out1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
+    const bool so = out_layout == ZLFFT_SOA;
+    const Buf bi{{in, nullptr}, {ss * n, 0}};
+    const Buf bo{{out0, so ? out1 : nullptr}, {so ? ss * nb : 2 * ss * nb, so ? ss * nb : 0}};
+    return run_host(p, bi, bo, batch, [&](const void* i0, const void*, void* o0, void* o1, size_t cnt, hipStream_t s) {
+        return zlfft_rfft_forward_exec(p, i0, out_layout, o0, o1, cnt, s);
     });
 }
 
 int zlfft_rfft_forward(zlfft_plan p, const void* in, int out_layout, void* out0, void* out1) {
+    return zlfft_rfft_forward_batch(p, in, out_layout, out0, out1, 1);
+}
+
+int zlfft_rfft_forward_sqr_mag_batch(zlfft_plan p, const void* in, void* out, size_t batch) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
-    if (!valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
-    if (!in || !out0 || (out_layout == ZLFFT_SOA && !out1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    if (batch == 0) return ZLFFT_OK;
+    if (!in || !out) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
     const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
-    const Buf bi{in, nullptr, ss * n, 0};
-    const Buf bo{out0, out_layout == ZLFFT_SOA ? out1 : nullptr, out_layout == ZLFFT_AOS ? 2 * ss * nb : ss * nb,
-                 out_layout == ZLFFT_SOA ? ss * nb : 0};
-    return run_sync(p, bi, bo, [&](const void* i0, const void*, void* o0, void* o1, hipStream_t s) {
-        return zlfft_rfft_forward_exec(p, i0, out_layout, o0, o1, 1, s);
+    const Buf bi{{in, nullptr}, {ss * n, 0}};
+    const Buf bo{{out, nullptr}, {ss * nb, 0}};
+    return run_host(p, bi, bo, batch, [&](const void* i0, const void*, void* o0, void*, size_t cnt, hipStream_t s) {
+        return zlfft_rfft_forward_sqr_mag_exec(p, i0, o0, cnt, s);
     });
 }
 
 int zlfft_rfft_forward_sqr_mag(zlfft_plan p, const void* in, void* out) {
+    return zlfft_rfft_forward_sqr_mag_batch(p, in, out, 1);
+}
+
+int zlfft_rfft_backward_batch(zlfft_plan p, int in_layout, const void* in0, const void* in1, void* out, size_t batch) {
     if (int rc = check_plan(p)) return rc;
     if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
-    if (!in || !out) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    if (!valid_layout(in_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch == 0) return ZLFFT_OK;
+    if (!in0 || !out || (in_layout == ZLFFT_SOA && !in1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
     const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
-    const Buf bi{in, nullptr, ss * n, 0};
-    const Buf bo{out, nullptr, ss * nb, 0};
-    return run_sync(p, bi, bo, [&](const void* i0, const void*, void* o0, void*, hipStream_t s) {
-        return zlfft_rfft_forward_sqr_mag_exec(p, i0, o0, 1, s);
+    const bool si = in_layout == ZLFFT_SOA;
+    const Buf bi{{in0, si ? in1 : nullptr}, {si ? ss * nb : 2 * ss * nb, si ? ss * nb : 0}};
+    const Buf bo{{out, nullptr}, {ss * n, 0}};
+    return run_host(p, bi, bo, batch, [&](const void* i0, const void* i1, void* o0, void*, size_t cnt, hipStream_t s) {
+        return zlfft_rfft_backward_exec(p, in_layout, i0, i1, o0, cnt, s);
     });
 }
 
 int zlfft_rfft_backward(zlfft_plan p, int in_layout, const void* in0, const void* in1, void* out) {
-    if (int rc = check_plan(p)) return rc;
-    if (p->kind != ZLFFT_KIND_RFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not an RFFT plan");
-    if (!valid_layout(in_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
-    if (!in0 || !out || (in_layout == ZLFFT_SOA && !in1)) return fail(ZLFFT_ERR_NULL, "NULL data pointer");
-    const size_t n = (size_t)1 << p->order, nb = n / 2 + 1, ss = p->ss;
-    const Buf bi{in0, in_layout == ZLFFT_SOA ? in1 : nullptr, in_layout == ZLFFT_AOS ? 2 * ss * nb : ss * nb,
-                 in_layout == ZLFFT_SOA ? ss * nb : 0};
-    const Buf bo{out, nullptr, ss * n, 0};
-    return run_sync(p, bi, bo, [&](const void* i0, const void* i1, void* o0, void*, hipStream_t s) {
-        return zlfft_rfft_backward_exec(p, in_layout, i0, i1, o0, 1, s);
-    });
+    return zlfft_rfft_backward_batch(p, in_layout, in0, in1, out, 1);
 }
 
 int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const void* window, size_t nframes, int output,
