@@ -421,7 +421,8 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
     const size_t groups = (batch + e.tpw - 1) / e.tpw;
     if (groups > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
     void* params[] = {&a};
-    const int resident = e.stream_fn[0] ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
+    static const bool no_stream = getenv("ZLFFT_NO_STREAM") != nullptr;  // tuning: LDS kernel only
+    const int resident = (e.stream_fn[0] && !no_stream) ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
     if (resident > 0 && groups > (size_t)resident) {
         // persistent streaming kernel: one workgroup per resident slot
         a.tw = p->d_img;
