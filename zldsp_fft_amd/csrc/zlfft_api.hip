@@ -545,7 +545,25 @@ int ensure_slot(zlfft_plan p, int i, size_t bytes) {
 // transferred directly; device arrays are used in place.  Multi-pass plans
 // share one workspace, so they use one slot.
 template <class Body>
+int run_host_chunks(zlfft_plan p, Buf in, Buf out, size_t batch, Body body);
+
+// run_host_chunks, and on failure waits for whatever it already queued, so
+// that no copy into the staging outlives the call
+template <class Body>
 int run_host(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
+    const int rc = run_host_chunks(p, in, out, batch, body);
+    if (rc != ZLFFT_OK) {
+        for (auto& sl : p->slot) {
+            if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+            if (sl.stream_out) (void)hipStreamSynchronize(sl.stream_out);
+        }
+        (void)hipStreamSynchronize(p->stream);
+    }
+    return rc;
+}
+
+template <class Body>
+int run_host_chunks(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
     MemKind kin[2] = {MEM_DEVICE, MEM_DEVICE}, kout[2] = {MEM_DEVICE, MEM_DEVICE};
     bool all_dev = true;
     for (int k = 0; k < 2; ++k) {
