@@ -402,6 +402,19 @@ int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, si
     return ZLFFT_OK;
 }
 
+// Streaming RFFT epilogue store policy from the output size of a call: cached
+// stores above ZLFFT_RFFT_CACHED_MB (default 64 MiB), non-temporal below
+// (measured: the 4096-frame analyzer batch is faster non-temporal, batches of
+// tens of thousands of frames cached).
+int rfft_cached_out(size_t out_bytes) {
+    static const size_t threshold = [] {
+        size_t mb = 64;
+        if (const char* e = getenv("ZLFFT_RFFT_CACHED_MB")) mb = (size_t)atol(e);
+        return mb << 20;
+    }();
+    return out_bytes > threshold ? 1 : 0;
+}
+
 int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
            size_t batch, int backward, void* stream) {
     if (p->macro) return launch_macro(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream);
@@ -427,6 +440,7 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
         // persistent streaming kernel: one workgroup per resident slot
         a.tw = p->d_img;
         a.sink = p->d_sink;
+        a.cached_out = rfft_cached_out(batch * (((size_t)1 << p->corder) + 1) * 8);
         ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident), dim3(e.wg),
                                params, 0, reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;

@@ -252,6 +252,7 @@ struct KArgs {
     long long hop;
     int db;            // R2C_SQR: write 10 log10(max(|X|^2, db_floor)) instead of |X|^2
     float db_floor;
+    int cached_out;    // streaming RFFT epilogue: cached instead of non-temporal stores
 };
 
 // Squared magnitude, or its level in dB when db is set (STFT front-end).
@@ -697,11 +698,16 @@ fft_stream_kernel(KArgs a) {
             for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, LOUT, eo + t + G::T * j, bwd ? swapri(v[j]) : v[j]);
         } else {
             // post-twiddle epilogue, identical to fft_lds_kernel's.  A frame's
-            // M + 1 outputs start off a cache-line boundary, so the stores go
-            // through the cache, which merges the lines neighbouring frames
-            // share (non-temporal stores measured up to 1.8x slower here).
-            auto rst_c = [&](void* o0, void* o1, int lay, long long e, float2 v) { st_c(o0, o1, lay, e, v); };
-            auto rst = [&](float* p, float v) { *p = v; };
+            // M + 1 outputs start off a cache-line boundary; with cached_out
+            // the stores go through the cache, which merges the lines that
+            // neighbouring frames share (the host picks it by batch size).
+            const bool cached = a.cached_out != 0;
+            auto rst_c = [&](void* o0, void* o1, int lay, long long e, float2 v) {
+                if (cached) st_c(o0, o1, lay, e, v); else st_c_nt(o0, o1, lay, e, v);
+            };
+            auto rst = [&](float* p, float v) {
+                if (cached) *p = v; else st_nt(p, v);
+            };
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
