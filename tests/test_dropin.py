@@ -38,3 +38,17 @@ def test_dropin_program_on_gpu():
     r = subprocess.run([BIN], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "DROPIN OK" in r.stdout
+
+
+def test_c_abi_header_is_plain_c():
+    """include/zlfft.h compiles as C99 and as C++ (the C-ABI the drop-in
+    classes and FFI callers use), and a C caller can name every entry point."""
+    import re
+    inc = os.path.join(ROOT, "include")
+    hdr = open(os.path.join(inc, "zlfft.h")).read()
+    names = sorted(set(re.findall(r"\b(zlfft_[a-z0-9_]+)\s*\(", re.sub(r"/\*.*?\*/", "", hdr, flags=re.S))))
+    code = '#include "zlfft.h"\n' + "typedef void (*fn_t)(void);\n" + "".join(f"fn_t p_{n} = (fn_t)&{n};\n" for n in names) + "int main(void) { return 0; }\n"
+    for lang, std in (("c", "-std=c99"), ("c++", "-std=c++17")):
+        r = subprocess.run(["gcc" if lang == "c" else "g++", std, "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                            "-pedantic", "-x", lang, "-I", inc, "-"], input=code, capture_output=True, text=True)
+        assert r.returncode == 0, (lang, r.stderr[-600:])
