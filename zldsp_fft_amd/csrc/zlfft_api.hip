@@ -276,15 +276,18 @@ int ensure_work(zlfft_plan p, size_t bytes) {
 
 // Multi-pass CFFT of order p->corder on `batch` transforms: pass 0 reads the
 // user layout lin, the last pass writes lout, intermediates ping-pong in work.
+// nt_io: bit 0 when the input, bit 1 when the output is the caller's buffer
+// (touched once: non-temporal access) rather than an internal one
 int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void* in1, int lout, void* out0,
-               void* out1, size_t batch, void* stream, void* work) {
+               void* out1, size_t batch, void* stream, void* work, int nt_io) {
     const long long n = 1ll << p->corder;
     void* tmp[2] = {work, static_cast<char*>(work) + (size_t)batch * (size_t)n * 2 * p->ss};
     const void* s0 = in0;
     const void* s1 = in1;
     for (int q = 0; q < p->mp; ++q) {
         const int role = q == 0 ? 0 : (q == p->mp - 1 ? 1 : 2);
-        const int lay = role == 0 ? lin : (role == 1 ? lout : zl::AOS);
+        const int lay = role == 0 ? ((nt_io & 1) ? lin : zl::kInternal)
+                                  : (role == 1 ? ((nt_io & 2) ? lout : zl::kInternal) : zl::AOS);
         const zl::MacroEntry& e = (p->f64 ? g_macro64 : g_macro)[p->logl[q]][role][lay];
         if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no HBM-pass kernel for this layout");
         void* d0 = role == 1 ? out0 : tmp[q & 1];
@@ -358,16 +361,18 @@ int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, s
     return ZLFFT_OK;
 }
 
+// user_in: in0 / in1 is the caller's input (0: frames the STFT just wrote)
 int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
-                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f) {
+                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f, int user_in = 1) {
     if (batch == 0) return ZLFFT_OK;
     if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
     void* work = p->d_work;
-    if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work);
+    if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work, 3);
     const size_t n = (size_t)1 << p->corder;
     void* z = static_cast<char*>(work) + (p->mp > 2 ? 2 : 1) * batch * n * 2 * p->ss;
     if (kind == zl::R2C || kind == zl::R2C_SQR) {
-        if (int rc = macro_cfft(p, 0, zl::AOS, in0, nullptr, zl::AOS, z, nullptr, batch, stream, work)) return rc;
+        if (int rc = macro_cfft(p, 0, zl::AOS, in0, nullptr, zl::AOS, z, nullptr, batch, stream, work, user_in))
+            return rc;
         return p->f64 ? rfft_post<double2>(p, kind, lout, z, out0, out1, batch, stream, db, db_floor)
                       : rfft_post<float2>(p, kind, lout, z, out0, out1, batch, stream, db, db_floor);
     }
@@ -375,7 +380,7 @@ int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, con
     if (int rc = p->f64 ? rfft_pre<double2>(p, lin, in0, in1, z, batch, stream)
                         : rfft_pre<float2>(p, lin, in0, in1, z, batch, stream))
         return rc;
-    return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work);
+    return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work, 2);
 }
 
 int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, size_t batch, int backward,
@@ -1040,7 +1045,7 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
                                                      static_cast<float*>(frames), (long long)hop, (long long)n,
                                                      (long long)nframes);
     ZL_HIP(hipGetLastError());
-    return launch_macro(p, kind, 0, lout, frames, nullptr, out0, out1, nframes, 0, stream, db, db_floor);
+    return launch_macro(p, kind, 0, lout, frames, nullptr, out0, out1, nframes, 0, stream, db, db_floor, 0);
 }
 
 int zlfft_plan_copy_rfft_twiddles(zlfft_plan p, float* host_out, size_t n_floats) {

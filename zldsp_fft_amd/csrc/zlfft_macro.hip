@@ -15,8 +15,10 @@ void ZL_CAT(zl_fill_macro_, ZL_LOGL)(zl::MacroTable& tab) {
     constexpr int C = cfg_macro_c(ZL_LOGL);
     tab[0][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS>();
     tab[0][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS>();
+    tab[0][kInternal] = make_macro_entry<GL, C, 1, 0, AOS, AOS, float2, 0>();
     tab[1][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS>();
     tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA>();
+    tab[1][kInternal] = make_macro_entry<GL, C, 0, 1, AOS, AOS, float2, 0>();
     tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS>();
 }
 
@@ -26,7 +28,9 @@ void ZL_CAT(zl_fill_macro64_, ZL_LOGL)(zl::MacroTable& tab) {
     constexpr int C = cfg_macro_c64(ZL_LOGL);
     tab[0][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS, double2>();
     tab[0][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS, double2>();
+    tab[0][kInternal] = make_macro_entry<GL, C, 1, 0, AOS, AOS, double2, 0>();
     tab[1][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2>();
     tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA, double2>();
+    tab[1][kInternal] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2, 0>();
     tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS, double2>();
 }

@@ -136,14 +136,17 @@ struct MacroEntry {
 };
 
 // [role][layout]: role 0 = first pass (layout = input), 1 = last pass
-// (layout = output), 2 = middle pass (layout unused, AoS)
-using MacroTable = MacroEntry[3][2];
+// (layout = output), 2 = middle pass (layout unused, AoS).  Layout index
+// kInternal (first / last pass only) is AoS on an internal buffer, cached
+// for the kernel that reads it next; AOS / SOA are the caller's buffers.
+constexpr int kInternal = 2;
+using MacroTable = MacroEntry[3][3];
 
-template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2>
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1>
 MacroEntry make_macro_entry() {
     static_assert((long long)sizeof(CX) * C * GL::LDSN <= 160 * 1024, "HBM pass exceeds the LDS");
     MacroEntry e{};
-    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT, CX>;
+    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT, CX, NT>;
     e.wg = C * GL::T;
     e.cols = C;
     e.logl = GL::logn;
