@@ -87,3 +87,44 @@ def test_stft_defaults_and_errors():
     with pytest.raises(zf.ZlfftError) as e:
         plan.stft(x, hop, torch.empty((nf, n // 2 + 1), device="cuda"), output=zf.STFT_DB, db_floor=0.0)
     assert e.value.code == 5
+
+
+@pytest.mark.parametrize("order,hop,nframes,windowed", [
+    (9, 77, 33001, True), (11, 333, 8193, True), (11, 512, 8195, False), (12, 1000, 4099, True),
+])
+def test_stft_large_batches(order, hop, nframes, windowed):
+    """Frame counts above any resident grid, so the persistent streaming form
+    runs (including a part-filled last group and odd hops, i.e. frames that
+    start off an 8-byte boundary).  Bins and powers are bit-identical to the
+    plain RFFT of the same frames cut and windowed on the device (one FP32
+    product per sample either way); dB levels match 10 log10 of those powers
+    to 1e-4 dB."""
+    n = 1 << order
+    nb = n // 2 + 1
+    plan = zf.RFFT(order)
+    g = torch.Generator(device="cuda").manual_seed(order)
+    x = torch.rand((nframes - 1) * hop + n, device="cuda", generator=g) * 2 - 1
+    w = torch.from_numpy(signals.hann(n).astype(np.float32)).cuda() if windowed else None
+    frames = x.unfold(0, n, hop)
+    frames = (frames * w if windowed else frames).contiguous()
+    ref = torch.empty((nframes, nb), dtype=torch.complex64, device="cuda")
+    plan.forward(frames, ref)
+    out = torch.full((nframes, nb), complex(np.nan, np.nan), dtype=torch.complex64, device="cuda")
+    assert plan.stft(x, hop, out, window=w, nframes=nframes) == nframes
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    re = torch.full((nframes, nb), np.nan, device="cuda")
+    im = torch.full((nframes, nb), np.nan, device="cuda")
+    plan.stft(x, hop, (re, im), window=w, nframes=nframes)
+    assert torch.equal(re, ref.real) and torch.equal(im, ref.imag)
+    pref = torch.empty((nframes, nb), device="cuda")
+    plan.forward_sqr_mag(frames, pref)
+    pw = torch.full((nframes, nb), np.nan, device="cuda")
+    plan.stft(x, hop, pw, window=w, output=zf.STFT_POWER, nframes=nframes)
+    assert torch.equal(pw, pref)
+    floor = 1e-9
+    db = torch.full((nframes, nb), np.nan, device="cuda")
+    plan.stft(x, hop, db, window=w, output=zf.STFT_DB, db_floor=floor, nframes=nframes)
+    dref = 10 * np.log10(np.maximum(pref.cpu().numpy().astype(np.float64), floor))
+    assert np.abs(db.cpu().numpy() - dref).max() <= 1e-4
+    print(f"stft streaming order {order} hop {hop} frames {nframes}: bit-identical bins/powers", flush=True)

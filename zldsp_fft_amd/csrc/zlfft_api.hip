@@ -225,7 +225,7 @@ struct zlfft_plan_s {
     float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
     float2* d_tw16 = nullptr;   // pass twiddles / 16 for the FP16 MFMA kernels
     int cus = 0;                // compute units of the device
-    int stream_blocks[4][2][2] = {};  // resident workgroups per CU of each streaming kernel
+    int stream_blocks[6][2][2] = {};  // resident workgroups per CU of each streaming kernel
     int mfma_blocks = 0;              // resident workgroups per CU of the persistent MFMA kernel
     void* d_rtw = nullptr;      // RFFT (wc, ws), complex type of the plan
     int n_rtw = 0;
@@ -420,6 +420,12 @@ int rfft_cached_out(size_t out_bytes) {
     return out_bytes > threshold ? 1 : 0;
 }
 
+// ZLFFT_NO_STREAM (tuning): LDS kernels only, no persistent streaming kernels
+bool no_stream_kernels() {
+    static const bool no_stream = getenv("ZLFFT_NO_STREAM") != nullptr;
+    return no_stream;
+}
+
 int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
            size_t batch, int backward, void* stream) {
     if (p->macro) return launch_macro(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream);
@@ -439,8 +445,7 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
     const size_t groups = (batch + e.tpw - 1) / e.tpw;
     if (groups > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
     void* params[] = {&a};
-    static const bool no_stream = getenv("ZLFFT_NO_STREAM") != nullptr;  // tuning: LDS kernel only
-    const int resident = (e.stream_fn[0] && !no_stream) ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
+    const int resident = (e.stream_fn[0] && !no_stream_kernels()) ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
     if (resident > 0 && groups > (size_t)resident) {
         // persistent streaming kernel: one workgroup per resident slot
         a.tw = p->d_img;
@@ -771,7 +776,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     if (!p->macro) {
         e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
         if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceGetAttribute"));
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 6; ++k)
             for (int i = 0; i < 2; ++i)
                 for (int o = 0; o < 2; ++o) {
                     const zl::KernelEntry& ke = (*p->tab)[k][i][o];
@@ -1023,6 +1028,17 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
         a.db = db;
         a.db_floor = db_floor;
         void* params[] = {&a};
+        const int fk = kind == zl::R2C ? zl::R2C_FRAMED : zl::R2C_SQR_FRAMED;
+        const int resident = (e.stream_fn[0] && !no_stream_kernels()) ? p->stream_blocks[fk][zl::AOS][lout] * p->cus : 0;
+        if (resident > 0 && groups > (size_t)resident) {
+            // persistent streaming kernel, as in launch()
+            a.tw = p->d_img;
+            a.sink = p->d_sink;
+            a.cached_out = rfft_cached_out(nframes * (((size_t)1 << p->corder) + 1) * 8);
+            ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[0]), dim3((unsigned)resident),
+                                   dim3(e.wg), params, 0, reinterpret_cast<hipStream_t>(stream)));
+            return ZLFFT_OK;
+        }
         ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params, 0,
                                reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;

@@ -664,13 +664,19 @@ __device__ __forceinline__ void run_pass_img(float2 (&v)[G::R], float2* slds, co
 // group the next group's input is already in flight (register prefetch with
 // non-temporal loads), and twiddles come from an LDS image loaded once, so no
 // global load waits behind the prefetch.  Same arithmetic as fft_lds_kernel.
+// FRAMED (R2C / R2C_SQR): the STFT front-end, as in fft_lds_kernel: frame f is
+// signal[f hop + n] (in0) times window[n] (in1, or none); the window sits in
+// LDS, the frames are read through the cache (frames overlap when hop < N;
+// measured equal or faster than non-temporal loads), and R2C_SQR applies the
+// dB option.
 #ifndef ZL_STREAM_MIN_WAVES
 #define ZL_STREAM_MIN_WAVES 1
 #endif
-template <class G, int KIND, int LIN, int LOUT, int BWD>
+template <class G, int KIND, int LIN, int LOUT, int BWD, int FRAMED = 0>
 __global__ void __launch_bounds__(G::WG) __attribute__((amdgpu_waves_per_eu(ZL_STREAM_MIN_WAVES, 8)))
 fft_stream_kernel(KArgs a) {
     static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR, "streaming kernel handles C2C and R2C");
+    static_assert(!FRAMED || KIND != C2C, "framed input is real");
     using TI = TwImage<G>;
     __shared__ float2 lds[G::TPW * G::LDSN];
     __shared__ float2 ltw[TI::total() > 0 ? TI::total() : 1];
@@ -695,16 +701,36 @@ fft_stream_kernel(KArgs a) {
     // waits for the prefetch never wait for the stores issued after it.
     const void* sk0 = a.sink;
     const void* sk1 = reinterpret_cast<const float*>(a.sink) + (1 << 14);
+    // prefetch of frame / transform f into p (the sink where !ok); a framed
+    // lane reads floats [0, 2 M) of the sink, M <= 2^14 (the sink holds 2^16)
+    auto prefetch = [&](float2 (&p)[G::R], long long f, bool ok) {
+        if constexpr (FRAMED) {
+            const float* x = static_cast<const float*>(ok ? a.in0 : sk0) + (ok ? f * a.hop : 0);
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) {
+                const int m = t + G::T * j;
+                p[j] = make_float2(x[2 * m], x[2 * m + 1]);
+            }
+        } else {
+            const void* b0 = ok ? a.in0 : sk0;
+            const void* b1 = ok ? a.in1 : sk1;
+            const long long eo = ok ? f * M : 0;
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt(b0, b1, lin, eo + t + G::T * j);
+        }
+    };
+    // the window in LDS, as M pairs (1 without a window: x * 1 = x); kept in
+    // registers instead it cost 32 VGPRs and a third of the occupancy
+    __shared__ float2 lwin[FRAMED ? M : 1];
+    if constexpr (FRAMED) {
+        const float* w = static_cast<const float*>(a.in1);
+        for (int e = tid; e < M; e += G::WG) lwin[e] = w ? make_float2(w[2 * e], w[2 * e + 1]) : make_float2(1.f, 1.f);
+    }
     float2 p[G::R];
     long long g = blockIdx.x;
     {
         const long long f = g * G::TPW + slot;
-        const bool ok = g < ngroups && f < a.batch;
-        const void* b0 = ok ? a.in0 : sk0;
-        const void* b1 = ok ? a.in1 : sk1;
-        const long long eo = ok ? f * M : 0;
-#pragma unroll
-        for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt(b0, b1, lin, eo + t + G::T * j);
+        prefetch(p, f, g < ngroups && f < a.batch);
     }
     // One group: take the prefetched input, prefetch the next group, transform,
     // store.  The first call is peeled off the loop so the loop is entered
@@ -719,14 +745,16 @@ fft_stream_kernel(KArgs a) {
         for (int j = 0; j < G::R; ++j) v[j] = bwd ? swapri(p[j]) : p[j];
         {
             const long long fn = (g + gridDim.x) * G::TPW + slot;
-            const bool ok = g + gridDim.x < ngroups && fn < a.batch;
-            const void* b0 = ok ? a.in0 : sk0;
-            const void* b1 = ok ? a.in1 : sk1;
-            const long long eo = ok ? fn * M : 0;
-#pragma unroll
-            for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt(b0, b1, lin, eo + t + G::T * j);
+            prefetch(p, fn, g + gridDim.x < ngroups && fn < a.batch);
         }
         __syncthreads();  // LDS image ready (first group) / previous group's LDS readers done
+        if constexpr (FRAMED) {
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) {
+                const float2 wj = lwin[t + G::T * j];
+                v[j] = make_float2(v[j].x * wj.x, v[j].y * wj.y);
+            }
+        }
         if constexpr (G::NP > 0) run_pass_img<G, 0>(v, slds, ltw, t);
         if constexpr (KIND == C2C) {
             void* o0 = active ? a.out0 : a.sink;
@@ -750,7 +778,9 @@ fft_stream_kernel(KArgs a) {
 #pragma unroll
             for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
             __syncthreads();
-            if (active) {
+            // pw: squared magnitude -> output value (|X|^2, or its dB level in
+            // the framed kernels; one branch per group, not per bin)
+            auto epilogue = [&](auto pw) {
                 const long long fb = f * (M + 1);
                 if (t == 0) {
                     const float2 z0 = slds[0];
@@ -760,8 +790,8 @@ fft_stream_kernel(KArgs a) {
                         rst_c(a.out0, a.out1, LOUT, fb + M, make_float2(ny, 0.f));
                     } else {
                         float* o = reinterpret_cast<float*>(a.out0);
-                        rst(o + fb, dc * dc);
-                        rst(o + fb + M, ny * ny);
+                        rst(o + fb, pw(dc * dc));
+                        rst(o + fb + M, pw(ny * ny));
                     }
                 }
                 for (int k = 1 + t; k <= M / 2; k += G::T) {
@@ -778,10 +808,16 @@ fft_stream_kernel(KArgs a) {
                         rst_c(a.out0, a.out1, LOUT, fb + (M - k), make_float2(xr_m, xi_m));
                     } else {
                         float* o = reinterpret_cast<float*>(a.out0);
-                        if (k != M - k) rst(o + fb + k, __builtin_fmaf(xr_k, xr_k, xi_k * xi_k));
-                        rst(o + fb + (M - k), __builtin_fmaf(xr_m, xr_m, xi_m * xi_m));
+                        if (k != M - k) rst(o + fb + k, pw(__builtin_fmaf(xr_k, xr_k, xi_k * xi_k)));
+                        rst(o + fb + (M - k), pw(__builtin_fmaf(xr_m, xr_m, xi_m * xi_m)));
                     }
                 }
+            };
+            if (active) {
+                if (FRAMED && KIND == R2C_SQR && a.db)
+                    epilogue([&](float q) { return power_out(q, 1, a.db_floor); });
+                else
+                    epilogue([](float q) { return q; });
             }
         }
     };
