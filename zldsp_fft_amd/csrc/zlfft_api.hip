@@ -175,11 +175,21 @@ __global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi
     out[idx] = make_float2((float)cs, (float)sn);
 }
 
-// FP16 MFMA kernels' twiddle table: every pass twiddle divided by 16 (exact).
-__global__ void scale_twiddles16(const float2* in, float2* out, int count) {
+// MFMA kernels' twiddle table (layout at kMfmaTwTotal): W_rw^{m k} for rows
+// m = 0..15, FP64 on the GPU, rounded to FP32, times `scale` (1, or 1/16 for
+// the FP16 kernels: exact), the same values as the pass table's.
+__global__ void gen_mfma_twiddles(float2* out, float scale) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= count) return;
-    out[idx] = make_float2(in[idx].x * 0.0625f, in[idx].y * 0.0625f);
+    if (idx >= zl::kMfmaTwTotal) return;
+    const bool p1 = idx < zl::kMfmaTwPass2;
+    const int j = p1 ? idx : idx - zl::kMfmaTwPass2;
+    const int lk = p1 ? 4 : 8;  // log2 of the row length
+    const long long m = j >> lk, k = j & ((1 << lk) - 1);
+    const long long rw = p1 ? 256 : 4096;
+    const long long e = (m * k) & (rw - 1);
+    double sn, cs;
+    sincospi(-2.0 * (double)e / (double)rw, &sn, &cs);
+    out[idx] = make_float2((float)cs * scale, (float)sn * scale);
 }
 
 // Macro twiddle table of an HBM pass: out[idx] = W_{2^log_lw}^{idx << shift},
@@ -223,7 +233,7 @@ struct zlfft_plan_s {
     void* d_tw = nullptr;       // pass twiddles, complex type of the plan
     float2* d_img = nullptr;    // LDS twiddle image for the streaming kernels
     float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
-    float2* d_tw16 = nullptr;   // pass twiddles / 16 for the FP16 MFMA kernels
+    float2* d_twm = nullptr;    // MFMA kernels' twiddle table (gen_mfma_twiddles)
     int cus = 0;                // compute units of the device
     int stream_blocks[6][2][2] = {};  // resident workgroups per CU of each streaming kernel
     int mfma_blocks = 0;              // resident workgroups per CU of the persistent MFMA kernel
@@ -397,8 +407,7 @@ int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, si
     zl::KArgs a{};
     a.in0 = in0;
     a.out0 = out0;
-    const bool f16 = p->precision == ZLFFT_PREC_F16_MFMA || p->precision == ZLFFT_PREC_F16_MFMA_SPLIT;
-    a.tw = f16 ? p->d_tw16 : p->d_tw;
+    a.tw = p->d_twm;
     a.sink = p->d_sink;
     a.batch = (long long)batch;
     void* params[] = {&a};
@@ -818,11 +827,11 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
             gen_pass(p->d_tw, off, ent.lr[q], ent.lw[q], cnt);
             off += (size_t)cnt;
         }
-        if (precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT) {
-            e = hipMalloc(&p->d_tw16, sizeof(float2) * (size_t)ent.tw_total);
+        if (mfma) {
+            e = hipMalloc(&p->d_twm, sizeof(float2) * (size_t)zl::kMfmaTwTotal);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
-            scale_twiddles16<<<(ent.tw_total + 255) / 256, 256, 0, p->stream>>>(static_cast<const float2*>(p->d_tw),
-                                                                                p->d_tw16, ent.tw_total);
+            const bool f16 = precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT;
+            gen_mfma_twiddles<<<(zl::kMfmaTwTotal + 255) / 256, 256, 0, p->stream>>>(p->d_twm, f16 ? 0.0625f : 1.0f);
         }
         e = hipGetLastError();
         if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
@@ -854,7 +863,7 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (p->d_tw) (void)hipFree(p->d_tw);
     if (p->d_img) (void)hipFree(p->d_img);
     if (p->d_sink) (void)hipFree(p->d_sink);
-    if (p->d_tw16) (void)hipFree(p->d_tw16);
+    if (p->d_twm) (void)hipFree(p->d_twm);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
     for (auto& sl : p->slot) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);

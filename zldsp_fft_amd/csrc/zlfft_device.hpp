@@ -304,6 +304,16 @@ __device__ __forceinline__ CX framed_pair(const KArgs& a, long long frame, long 
     return CxT<CX>::mk(x0, x1);
 }
 
+// Twiddle table of the MFMA 4096-point kernels, rows m = 0..15 so that every
+// lane addresses it from a base plus compile-time offsets: pass 1 holds
+// W_256^{m k} at [16 m + k] (k < 16), pass 2 W_4096^{m k} at
+// [kMfmaTwPass2 + 256 m + k] (k < 256); divided by 16 for the FP16 kernels.
+// (Factoring pass 2 as W_256^{m a} W_4096^{m b}, k = 16 a + b, cuts the table
+// to 4 KiB and allows 3 workgroups per CU, but its extra complex product per
+// value measured 1.5% slower.)
+constexpr int kMfmaTwPass2 = 256;
+constexpr int kMfmaTwTotal = kMfmaTwPass2 + 16 * 256;
+
 // size of KArgs::sink: two arrays of the largest LDS transform (2^14 complex)
 constexpr long long kSinkBytes = 2ll * 8 * (1 << 14);
 

@@ -10,11 +10,15 @@
 //   D lane l holds D[4 (l / 16) + r][l % 16], r < 4
 // A group of 16 butterflies b (columns) is one product.  The K index
 // interleaves real and imaginary parts, k = 2 m + c, so lane l's B operand is
-// the 4 complex inputs m = 4 (l / 16) .. + 3 of butterfly l % 16, and
-//   Yr = A_r B with A_r[q][2m] = Re F[q][m], A_r[q][2m+1] = -Im F[q][m]
-//   Yi = A_i B with A_i[q][2m] = Im F[q][m], A_i[q][2m+1] =  Re F[q][m]
-// for F[q][m] = exp(-2 pi i q m / 16).  Lane l then holds the 4 complex
-// outputs q = 4 (l / 16) + r of butterfly l % 16.
+// the 4 complex inputs m = 4 (l / 16) .. + 3 of butterfly l % 16.  The rows
+// interleave real and imaginary parts of the outputs too: with
+// F[q][m] = exp(-2 pi i q m / 16), A_lo holds outputs q = 0..7 and A_hi
+// q = 8..15 as
+//   A[2 q'][2m] = Re F[q][m],  A[2 q'][2m+1] = -Im F[q][m]     (Re y_q)
+//   A[2 q'+1][2m] = Im F[q][m], A[2 q'+1][2m+1] = Re F[q][m]   (Im y_q)
+// (q' = q mod 8), so lane l holds the complex outputs q = 2 (l / 16) and
+// 2 (l / 16) + 1 of each product, with the real and imaginary parts in
+// neighbouring accumulator registers, ready to store as pairs.
 #pragma once
 
 #include "zlfft_device.hpp"
@@ -151,32 +155,38 @@ __device__ __forceinline__ double cos16(int j) {
     return t[j & 15];
 }
 
-// A operands (DFT-16 matrix, split in NS parts) of this lane
+// A operands (DFT-16 matrix rows of outputs 0..7 and 8..15, split in NS
+// parts) of this lane
 template <int P, int NS>
-__device__ __forceinline__ void dft16_operands(int lane, typename MfOp<P>::V (&ar)[NS],
-                                               typename MfOp<P>::V (&ai)[NS]) {
-    const int q = lane & 15, h = lane >> 4;
-    float fr[8], fi[8];
+__device__ __forceinline__ void dft16_operands(int lane, typename MfOp<P>::V (&alo)[NS],
+                                               typename MfOp<P>::V (&ahi)[NS]) {
+    const int row = lane & 15, h = lane >> 4;
+    const int c = row & 1;  // 0: real part of the output, 1: imaginary part
+    float flo[8], fhi[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int m = 4 * h + (e >> 1);
-        const int qm = (q * m) & 15;
-        const double re = cos16(qm), im = -cos16(qm - 4);  // Im exp(-2 pi i qm/16) = -sin = -cos(x - pi/2)
-        // exact split of the FP64 value: parts of (re, im) in the operand type
-        fr[e] = (e & 1) ? (float)(-im) : (float)re;
-        fi[e] = (e & 1) ? (float)re : (float)im;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            const int q = 8 * half + (row >> 1);
+            const int qm = (q * m) & 15;
+            const double re = cos16(qm), im = -cos16(qm - 4);  // Im exp(-2 pi i qm/16) = -sin = -cos(x - pi/2)
+            // exact split of the FP64 value: parts in the operand type
+            const float v = c == 0 ? ((e & 1) ? (float)(-im) : (float)re) : ((e & 1) ? (float)re : (float)im);
+            (half ? fhi : flo)[e] = v;
+        }
     }
-    split8<P, NS, true>(fr, ar);
-    split8<P, NS, true>(fi, ai);
+    split8<P, NS, true>(flo, alo);
+    split8<P, NS, true>(fhi, ahi);
 }
 
 // Y = F X for one group of 16 butterflies; x: this lane's 4 complex inputs
-// (m = 4h + i), returns this lane's 4 complex outputs (q = 4h + r).
+// (m = 4h + i), returns this lane's 4 complex outputs y[r], q = mf_out_q(h, r).
 // SCALED (FP16 only): the inputs are multiplied by the power of two s first,
 // inside the split.
 template <int P, int NS, bool SCALED = false>
-__device__ __forceinline__ void dft16_mfma(const float2 (&x)[4], float2 (&y)[4], const typename MfOp<P>::V (&ar)[NS],
-                                           const typename MfOp<P>::V (&ai)[NS], float s = 1.0f) {
+__device__ __forceinline__ void dft16_mfma(const float2 (&x)[4], float2 (&y)[4], const typename MfOp<P>::V (&alo)[NS],
+                                           const typename MfOp<P>::V (&ahi)[NS], float s = 1.0f) {
     using OP = MfOp<P>;
     float v[8];
 #pragma unroll
@@ -191,31 +201,25 @@ __device__ __forceinline__ void dft16_mfma(const float2 (&x)[4], float2 (&y)[4],
     } else {
         split8<P, NS>(v, bp);
     }
-    zl_f4 cr = {0.f, 0.f, 0.f, 0.f}, ci = {0.f, 0.f, 0.f, 0.f};
+    zl_f4 dlo = {0.f, 0.f, 0.f, 0.f}, dhi = {0.f, 0.f, 0.f, 0.f};
     // terms of total split order < NS, smallest first
 #pragma unroll
     for (int ord = NS - 1; ord >= 0; --ord) {
 #pragma unroll
         for (int ia = 0; ia <= ord; ++ia) {
             const int ib = ord - ia;
-            cr = OP::mma(ar[ia], bp[ib], cr);
-            ci = OP::mma(ai[ia], bp[ib], ci);
+            dlo = OP::mma(alo[ia], bp[ib], dlo);
+            dhi = OP::mma(ahi[ia], bp[ib], dhi);
         }
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = make_float2(cr[r], ci[r]);
+    y[0] = make_float2(dlo[0], dlo[1]);
+    y[1] = make_float2(dlo[2], dlo[3]);
+    y[2] = make_float2(dhi[0], dhi[1]);
+    y[3] = make_float2(dhi[2], dhi[3]);
 }
 
-// W_{16 w}^{m k} of pass P of geometry G from the LDS copy of the plan's pass
-// twiddles ([tw_off(P) + (m - 1) w + k]); for FP16 the table holds W / 16
-// and m = 0 gives 1 / 16 likewise
-template <class G, int P, int PREC>
-__device__ __forceinline__ float2 mf_twiddle(const float2* ltw, int m, int k) {
-    constexpr int w = 1 << G::lw(P);
-    const float2 t = ltw[G::tw_off(P) + (m > 0 ? m - 1 : 0) * w + k];
-    return m > 0 ? t : make_float2(PREC == MF_F16 ? 0.0625f : 1.f, 0.f);
-}
-constexpr int mf_tw_total() { return Geo<12, 4, 256>::tw_total(); }
+// output index q of y[r] in lane group h (dft16_mfma): 2 h + (r & 1) + 8 (r >> 1)
+constexpr int mf_out_dq(int r) { return (r & 1) + 8 * (r >> 1); }
 
 // largest |component| of this wave's inputs into wmax[wave]: bit patterns
 // with the sign cleared compare as unsigned integers in the order of the
@@ -258,11 +262,24 @@ __device__ __forceinline__ int mfma_scale_exponent(const float* wmax) {
 // that synchronisation.
 template <int P, int NS, int BWD, bool MAXDONE>
 __device__ __forceinline__ void mfma4096_transform(float2 (&x)[4][4], float2* lds, const float2* ltw, float* wmax,
-                                                   const typename MfOp<P>::V (&ar)[NS],
-                                                   const typename MfOp<P>::V (&ai)[NS], float2* out) {
-    using G = Geo<12, 4, 256>;
+                                                   const typename MfOp<P>::V (&alo)[NS],
+                                                   const typename MfOp<P>::V (&ahi)[NS], float2* out) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = lane & 15, h = lane >> 4;
+    // LDS and twiddle addresses as a per-lane base plus compile-time offsets
+    // (b < 16, so the one-per-16 padding e + (e >> 4) separates):
+    //  pass input e = i + 256 m, i = 16 (4 wave + gg) + b, m = 4 h + mi:
+    //    pad(e) = b + 17 (4 wave + gg) + 272 m
+    //  outputs q = 2 h + mf_out_dq(r):
+    //  pass 0 output 16 i + q:                        pad = 17 i + q
+    //  pass 1 output 256 (4 wave + gg) + b + 16 q:
+    //    pad = 272 (4 wave + gg) + b + 17 q
+    //  twiddles: pass 1 [16 m + b], pass 2 [kMfmaTwPass2 + 256 m + i]
+    const int rd = b + 68 * wave + 1088 * h;     // + 17 gg + 272 mi
+    const int wr0 = 1088 * wave + 17 * b + 2 * h;  // + 272 gg + mf_out_dq(r)
+    const int wr1 = 1088 * wave + b + 34 * h;      // + 272 gg + 17 mf_out_dq(r)
+    const int tw1 = 64 * h + b;                    // + 16 mi
+    const int tw2 = kMfmaTwPass2 + 1024 * h + 64 * wave + b;  // + 256 mi + 16 gg
     float2 y[4][4];
     int sh = 0;
     float s = 1.0f;  // 2^sh, applied inside the first pass's split
@@ -284,70 +301,73 @@ __device__ __forceinline__ void mfma4096_transform(float2 (&x)[4][4], float2* ld
         }
     }
     // net power of two to undo at the store: the input scale and two 1/16
-    // twiddle factors; a float multiplier where 2^unscale is a normal float
+    // twiddle factors (one v_ldexp_f32 per value, exact for every result)
     const int unscale = P == MF_F16 ? 8 - sh : 0;
-    const bool mul_unscale = unscale >= -126 && unscale <= 127;
-    const float us = mul_unscale ? ldexpf(1.0f, unscale) : 1.0f;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         if (p > 0) {
 #pragma unroll
             for (int gg = 0; gg < 4; ++gg) {
-                const int i = 16 * (wave * 4 + gg) + b;
 #pragma unroll
                 for (int mi = 0; mi < 4; ++mi) {
-                    const int m = 4 * h + mi;
-                    const float2 t = lds[G::pad(i + 256 * m)];
-                    const float2 w = p == 1 ? mf_twiddle<G, 1, P>(ltw, m, i & 15) : mf_twiddle<G, 2, P>(ltw, m, i);
+                    const float2 t = lds[rd + 17 * gg + 272 * mi];
+                    const float2 w = p == 1 ? ltw[tw1 + 16 * mi] : ltw[tw2 + 256 * mi + 16 * gg];
                     x[gg][mi] = cmul(t, w);
                 }
             }
         }
         if (P == MF_F16 && p == 0) {
 #pragma unroll
-            for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS, P == MF_F16>(x[gg], y[gg], ar, ai, s);
+            for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS, P == MF_F16>(x[gg], y[gg], alo, ahi, s);
         } else {
 #pragma unroll
-            for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS>(x[gg], y[gg], ar, ai);
+            for (int gg = 0; gg < 4; ++gg) dft16_mfma<P, NS>(x[gg], y[gg], alo, ahi);
         }
         if (p < 2) {
             __syncthreads();  // all reads of the previous LDS contents are done
-            const int lw = 4 * p;
 #pragma unroll
             for (int gg = 0; gg < 4; ++gg) {
-                const int i = 16 * (wave * 4 + gg) + b;
-                const int base = ((i >> lw) << (lw + 4)) + (i & ((1 << lw) - 1));
 #pragma unroll
-                for (int r = 0; r < 4; ++r) lds[G::pad(base + ((4 * h + r) << lw))] = y[gg][r];
+                for (int r = 0; r < 4; ++r) {
+                    if (p == 0)
+                        lds[wr0 + 272 * gg + mf_out_dq(r)] = y[gg][r];
+                    else
+                        lds[wr1 + 272 * gg + 17 * mf_out_dq(r)] = y[gg][r];
+                }
             }
             __syncthreads();
         } else {
+            // output i + 256 q, i = 64 wave + 16 gg + b, q = 2 h + mf_out_dq(r);
+            // byte offsets as 32-bit unsigned values
+            char* obase = reinterpret_cast<char*>(out);
+            const unsigned so = 8u * (64u * wave + b + 512u * h);
 #pragma unroll
             for (int gg = 0; gg < 4; ++gg) {
-                const int i = 16 * (wave * 4 + gg) + b;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     float2 o = y[gg][r];
-                    if constexpr (P == MF_F16)
-                        o = mul_unscale ? make_float2(o.x * us, o.y * us)
-                                        : make_float2(ldexpf(o.x, unscale), ldexpf(o.y, unscale));
-                    st_nt(out + i + 256 * (4 * h + r), BWD ? swapri(o) : o);
+                    if constexpr (P == MF_F16) o = make_float2(ldexpf(o.x, unscale), ldexpf(o.y, unscale));
+                    st_nt(reinterpret_cast<float2*>(obase + (so + 8u * (16u * gg + 256u * mf_out_dq(r)))),
+                          BWD ? swapri(o) : o);
                 }
             }
         }
     }
 }
 
-// this lane's inputs of transform `in` (4096 complex, AoS)
+// this lane's inputs of transform `in` (4096 complex, AoS; the same pointer
+// in every lane)
 __device__ __forceinline__ void mfma4096_load(float2 (&x)[4][4], const float2* in) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = lane & 15, h = lane >> 4;
+    const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned b = lane & 15, h = lane >> 4;
+    // byte offsets as 32-bit unsigned values: scalar base + vector offset
+    const char* base = reinterpret_cast<const char*>(in);
+    const unsigned lo = 8u * (64u * wave + b + 1024u * h);
 #pragma unroll
-    for (int gg = 0; gg < 4; ++gg) {
-        const int i = 16 * (wave * 4 + gg) + b;
+    for (int gg = 0; gg < 4; ++gg)
 #pragma unroll
-        for (int mi = 0; mi < 4; ++mi) x[gg][mi] = ld_nt(in + i + 256 * (4 * h + mi));
-    }
+        for (int mi = 0; mi < 4; ++mi)
+            x[gg][mi] = ld_nt(reinterpret_cast<const float2*>(base + (lo + 8u * (16u * gg + 256u * mi))));
 }
 
 // 4096-point CFFT, AoS in and out, one transform per workgroup.
@@ -355,12 +375,12 @@ template <int P, int NS, int BWD>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFMA_STREAM_WAVES))) fft4096_mfma_kernel(KArgs a) {
     using G = Geo<12, 4, 256>;
     __shared__ float2 lds[G::LDSN];
-    __shared__ float2 ltw[mf_tw_total()];
+    __shared__ float2 ltw[kMfmaTwTotal];
     __shared__ float wmax[4];
     const int tid = threadIdx.x;
-    typename MfOp<P>::V ar[NS], ai[NS];
-    dft16_operands<P, NS>(tid & 63, ar, ai);
-    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
+    typename MfOp<P>::V alo[NS], ahi[NS];
+    dft16_operands<P, NS>(tid & 63, alo, ahi);
+    for (int e = tid; e < kMfmaTwTotal; e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
     const long long f = blockIdx.x;
     float2 x[4][4];
     mfma4096_load(x, reinterpret_cast<const float2*>(a.in0) + f * 4096);
@@ -371,7 +391,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFM
             for (int mi = 0; mi < 4; ++mi) x[gg][mi] = swapri(x[gg][mi]);
     }
     __syncthreads();  // LDS twiddles
-    mfma4096_transform<P, NS, BWD, false>(x, lds, ltw, wmax, ar, ai, reinterpret_cast<float2*>(a.out0) + f * 4096);
+    mfma4096_transform<P, NS, BWD, false>(x, lds, ltw, wmax, alo, ahi, reinterpret_cast<float2*>(a.out0) + f * 4096);
 }
 
 // Persistent form: one workgroup per resident slot loops over transforms
@@ -385,19 +405,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZL_MFM
 fft4096_mfma_stream_kernel(KArgs a) {
     using G = Geo<12, 4, 256>;
     __shared__ float2 lds[G::LDSN];
-    __shared__ float2 ltw[mf_tw_total()];
+    __shared__ float2 ltw[kMfmaTwTotal];
     __shared__ float wmax[4];
     const int tid = threadIdx.x;
-    typename MfOp<P>::V ar[NS], ai[NS];
-    dft16_operands<P, NS>(tid & 63, ar, ai);
-    for (int e = tid; e < mf_tw_total(); e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
+    typename MfOp<P>::V alo[NS], ahi[NS];
+    dft16_operands<P, NS>(tid & 63, alo, ahi);
+    for (int e = tid; e < kMfmaTwTotal; e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
     const float2* in = reinterpret_cast<const float2*>(a.in0);
     const float2* sink = reinterpret_cast<const float2*>(a.sink);
+    // source of transform t: the input, or the sink past the batch end.  A
+    // 32-bit unsigned compare (batch < 2^31, checked by the host) stays in
+    // scalar registers, so the pointer select does too.
+    const unsigned nbatch = (unsigned)a.batch;
+    auto src = [&](long long t) { return (unsigned)t < nbatch ? in + t * 4096 : sink; };
     long long f = blockIdx.x;
     // two prefetch buffers used in turn (the loop body is unrolled twice), so
     // the prefetched registers are never copied
     float2 pa[4][4], pb[4][4];
-    mfma4096_load(pa, f < a.batch ? in + f * 4096 : sink);
+    mfma4096_load(pa, src(f));
     // FP16: the per-wave maxima of a transform are taken from the prefetch
     // registers at the end of the previous iteration, so the synchronisation
     // at the top of the next one publishes them and the transform needs none
@@ -410,10 +435,9 @@ fft4096_mfma_stream_kernel(KArgs a) {
 #pragma unroll
                 for (int mi = 0; mi < 4; ++mi) cur[gg][mi] = swapri(cur[gg][mi]);
         }
-        const long long fn = f + gridDim.x;
-        mfma4096_load(nxt, fn < a.batch ? in + fn * 4096 : sink);
+        mfma4096_load(nxt, src(f + gridDim.x));
         __syncthreads();  // LDS twiddles and maxima ready / previous transform's LDS readers done
-        mfma4096_transform<P, NS, BWD, true>(cur, lds, ltw, wmax, ar, ai,
+        mfma4096_transform<P, NS, BWD, true>(cur, lds, ltw, wmax, alo, ahi,
                                              reinterpret_cast<float2*>(a.out0) + f * 4096);
         // every wave read wmax before the transform's first LDS exchange barrier
         if constexpr (P == MF_F16) mfma_wave_absmax(nxt, wmax);
