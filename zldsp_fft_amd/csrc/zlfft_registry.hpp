@@ -38,15 +38,16 @@ template <class G, int KIND, int LIN, int LOUT, int FRAMED = 0>
 void add_stream_variant(KernelEntry& e) {
     constexpr int elem = (KIND == C2C && (LIN == SOA || LOUT == SOA)) ? 4 : 8;
     constexpr bool direct = G::T * elem >= 128 || G::TPW == 1;
-    constexpr long long stream_lds = 8ll * (G::TPW * G::LDSN + TwImage<G>::total() + (KIND == C2C ? 1 : G::N / 2) +
+    using TI = TwImage<G, cfg_tw_direct_max(G::logn, KIND)>;
+    constexpr long long stream_lds = 8ll * (G::TPW * G::LDSN + TI::total() + (KIND == C2C ? 1 : G::N / 2) +
                                             (FRAMED ? G::N : 0));
     if constexpr (KIND != C2R && direct && G::NP >= 2 && stream_lds <= 160 * 1024) {
         e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED>;
         if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1>;
-        e.img_total = TwImage<G>::total();
+        e.img_total = TI::total();
         for (int p = 1; p < G::NP && p < 8; ++p) {
-            e.img_lo[p] = TwImage<G>::lo_n(p);
-            e.img_hi[p] = TwImage<G>::hi_n(p);
+            e.img_lo[p] = TI::lo_n(p);
+            e.img_hi[p] = TI::hi_n(p);
         }
     }
 }
