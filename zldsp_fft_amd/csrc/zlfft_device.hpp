@@ -606,9 +606,10 @@ __device__ __forceinline__ void st_c_nt(void* p0, void* p1, int lay, long long e
 // Largest pass width whose twiddles the streaming kernel's LDS image holds
 // directly (above it: two factors).  The direct table costs LDS, but saves a
 // complex product per value: measured 1.2% faster for the 4096-point C2C
-// batch (2 workgroups per CU instead of 3); slower for the RFFT and STFT
-// kernels, whose LDS also holds the RFFT twiddles and the window.
-constexpr int cfg_tw_direct_max(int logn, int kind) { return kind == C2C && logn == 12 ? 4096 : 16; }
+// batch (2 workgroups per CU instead of 3) and 1-3% at orders 9-10; slower
+// for the RFFT and STFT kernels, whose LDS also holds the RFFT twiddles and
+// the window, and for orders 13-14, where the table outgrows the LDS budget.
+constexpr int cfg_tw_direct_max(int logn, int kind) { return kind == C2C && logn >= 9 && logn <= 12 ? 4096 : 16; }
 
 // LDS image of the Stockham twiddles of geometry G.  Pass p (radix r, width w)
 // needs W_{r w}^{m k}, m = 1..r-1, k < w.  For w <= DMAX the image holds them
