@@ -128,6 +128,7 @@ __device__ __forceinline__ CX w16(CX a) {
 //  r = 8: two radix-4 plus W_8 via 1/sqrt(2)
 //         (reference src/common/zldsp_fft_common_radix.hpp:499-560)
 //  r = 16: 4 x 4: DFT-4 over x[j + 4m], twiddle W16^{j q}, DFT-4 over j.
+//  r = 32: 2 x 16 (the radix-1024 HBM passes: 32 x 32).
 template <int r, class CX>
 __device__ __forceinline__ void dft(CX* x) {
     if constexpr (r == 1) {
@@ -182,6 +183,41 @@ __device__ __forceinline__ void dft(CX* x) {
             x[q + 4] = c[1];
             x[q + 8] = c[2];
             x[q + 12] = c[3];
+        }
+    } else if constexpr (r == 32) {
+        // 2 x 16: DFT-16 over the even and the odd inputs, W_32^k on the odd
+        // half, then radix-2 butterflies
+        using T = CxT<CX>;
+        using S = typename T::S;
+        // cos(pi j / 16), j = 0..8; W_32^k = (cos(pi k / 16), -sin(pi k / 16))
+        constexpr double c[9] = {1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+                                 0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173,
+                                 0.19509032201612826785, 0.0};
+        CX e[16], o[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            e[j] = x[2 * j];
+            o[j] = x[2 * j + 1];
+        }
+        dft<16>(e);
+        dft<16>(o);
+#pragma unroll
+        for (int k = 1; k < 16; ++k) {
+            if (k == 8) {
+                o[k] = mul_mi(o[k]);
+            } else if (k == 4) {
+                o[k] = w16<2>(o[k]);
+            } else if (k == 12) {
+                o[k] = w16<6>(o[k]);
+            } else {
+                const double cr = k < 8 ? c[k] : -c[16 - k], ci = k < 8 ? c[8 - k] : c[k - 8];  // cos, sin of pi k/16
+                o[k] = cmul(o[k], T::mk((S)cr, (S)-ci));
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            x[q] = cadd(e[q], o[q]);
+            x[q + 16] = csub(e[q], o[q]);
         }
     } else {
         static_assert(r == 1, "unsupported radix");
