@@ -315,12 +315,15 @@ struct KArgs {
 };
 
 // Squared magnitude, or its level in dB when db is set (STFT front-end).
+// FP32: 10 log10(q) = (10 log10 2) log2(q) with the hardware base-2 logarithm
+// (v_log_f32) for normal q, the library log10f for a subnormal floor; the
+// tests hold the levels to 1e-4 dB of the FP64 value of the same power.
 template <class S>
 __device__ __forceinline__ S power_out(S p, int db, float db_floor) {
     if (!db) return p;
     const S q = p > (S)db_floor ? p : (S)db_floor;
     if constexpr (sizeof(S) == 4)
-        return 10.0f * log10f(q);
+        return q >= 1.17549435e-38f ? 3.01029995663981195f * __builtin_amdgcn_logf(q) : 10.0f * log10f(q);
     else
         return 10.0 * log10(q);
 }
