@@ -231,8 +231,7 @@ struct zlfft_plan_s {
     size_t ss = 4;              // bytes per real scalar (4 or 8)
     const zl::OrderTable* tab = nullptr;  // LDS kernels (corder <= kMaxLdsOrder[64])
     void* d_tw = nullptr;       // pass twiddles, complex type of the plan
-    float2* d_img = nullptr;    // LDS twiddle image of the C2C streaming kernels
-    float2* d_img_r = nullptr;  // the same for the real kinds (d_img where the layouts agree)
+    float2* d_img = nullptr;    // LDS twiddle image of the streaming kernels of the plan's kind
     float2* d_sink = nullptr;   // zl::kSinkBytes of scratch for their idle lanes
     float2* d_twm = nullptr;    // MFMA kernels' twiddle table (gen_mfma_twiddles)
     int cus = 0;                // compute units of the device
@@ -458,7 +457,7 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
     const int resident = (e.stream_fn[0] && !no_stream_kernels()) ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
     if (resident > 0 && groups > (size_t)resident) {
         // persistent streaming kernel: one workgroup per resident slot
-        a.tw = kind == zl::C2C ? p->d_img : p->d_img_r;
+        a.tw = p->d_img;
         a.sink = p->d_sink;
         a.cached_out = rfft_cached_out(batch * (((size_t)1 << p->corder) + 1) * 8);
         ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident), dim3(e.wg),
@@ -725,6 +724,9 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     p->macro = corder > (size_t)(f64 ? zl::kMaxLdsOrder64 : zl::kMaxLdsOrder);
     p->tab = p->macro ? nullptr : f64 ? &g_tab64[corder] : &g_tab[corder];
     const size_t csz = 2 * p->ss;  // bytes per complex value
+    // the table entry whose twiddles the plan holds: CFFT plans launch only
+    // C2C kernels, RFFT plans only the real kinds (which share one geometry)
+    const int own_kind = kind == ZLFFT_KIND_RFFT ? zl::R2C : zl::C2C;
     // twiddle generators for the plan's complex type
     auto gen_pass = [&](void* out, size_t off, int lr, int lw, int cnt) {
         const unsigned g = (unsigned)((cnt + 255) / 256);
@@ -806,33 +808,23 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
             e = hipMalloc(&p->d_sink, (size_t)zl::kSinkBytes);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "sink hipMalloc failed"));
         }
-        // one image per layout: the C2C kernels' and the real kinds' (which
-        // differ where the C2C kernel keeps a direct table)
-        auto make_image = [&](const zl::KernelEntry& k, float2** dst) -> int {
-            if (k.img_total <= 0) return ZLFFT_OK;
-            if (hipMalloc(dst, sizeof(float2) * (size_t)k.img_total) != hipSuccess)
-                return fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed");
+        // the LDS twiddle image of the plan's kind: C2C for CFFT plans, the
+        // real kinds' (one geometry and image for all of them) for RFFT plans
+        const zl::KernelEntry& ki = (*p->tab)[own_kind][0][0];
+        if (ki.img_total > 0) {
+            e = hipMalloc(&p->d_img, sizeof(float2) * (size_t)ki.img_total);
+            if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
             size_t off = 0;
-            for (int q = 1; q < k.np; ++q) {
-                const int cnt = k.img_lo[q] + k.img_hi[q];
-                gen_image_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(*dst + off, k.lr[q], k.lw[q],
-                                                                              k.img_lo[q], k.img_hi[q]);
+            for (int q = 1; q < ki.np; ++q) {
+                const int cnt = ki.img_lo[q] + ki.img_hi[q];
+                gen_image_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_img + off, ki.lr[q], ki.lw[q],
+                                                                              ki.img_lo[q], ki.img_hi[q]);
                 off += (size_t)cnt;
             }
-            return ZLFFT_OK;
-        };
-        const zl::KernelEntry& kr = (*p->tab)[zl::R2C][0][0];
-        if (int rc = make_image(ke, &p->d_img)) return cleanup(rc);
-        bool same = kr.img_total == ke.img_total;
-        for (int q = 1; same && q < ke.np && q < 8; ++q)
-            same = kr.img_lo[q] == ke.img_lo[q] && kr.img_hi[q] == ke.img_hi[q];
-        if (same || !kr.stream_fn[0]) {
-            p->d_img_r = p->d_img;
-        } else if (int rc = make_image(kr, &p->d_img_r)) {
-            return cleanup(rc);
         }
     }
-    const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[zl::C2C][0][0];
+    // pass twiddles of the plan's kind (the LDS kernels')
+    const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[own_kind][0][0];
     if (ent.tw_total > 0) {
         e = hipMalloc(&p->d_tw, csz * (size_t)ent.tw_total);
         if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
@@ -876,7 +868,6 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (!p) return fail(ZLFFT_ERR_NULL, "plan is NULL");
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->d_tw) (void)hipFree(p->d_tw);
-    if (p->d_img_r && p->d_img_r != p->d_img) (void)hipFree(p->d_img_r);
     if (p->d_img) (void)hipFree(p->d_img);
     if (p->d_sink) (void)hipFree(p->d_sink);
     if (p->d_twm) (void)hipFree(p->d_twm);
@@ -1057,7 +1048,7 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
         const int resident = (e.stream_fn[0] && !no_stream_kernels()) ? p->stream_blocks[fk][zl::AOS][lout] * p->cus : 0;
         if (resident > 0 && groups > (size_t)resident) {
             // persistent streaming kernel, as in launch()
-            a.tw = p->d_img_r;
+            a.tw = p->d_img;
             a.sink = p->d_sink;
             a.cached_out = rfft_cached_out(nframes * (((size_t)1 << p->corder) + 1) * 8);
             ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[0]), dim3((unsigned)resident),
