@@ -408,9 +408,17 @@ fft4096_mfma_stream_kernel(KArgs a) {
     __shared__ float2 ltw[kMfmaTwTotal];
     __shared__ float wmax[4];
     const int tid = threadIdx.x;
+    // twiddle table: all loads first, then the first prefetch, then the LDS
+    // writes (one memory round trip at the start, not one per table row)
+    constexpr int NT = (kMfmaTwTotal + 255) / 256;
+    float2 twr[NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int e = tid + 256 * i;
+        if (e < kMfmaTwTotal) twr[i] = static_cast<const float2*>(a.tw)[e];
+    }
     typename MfOp<P>::V alo[NS], ahi[NS];
     dft16_operands<P, NS>(tid & 63, alo, ahi);
-    for (int e = tid; e < kMfmaTwTotal; e += 256) ltw[e] = static_cast<const float2*>(a.tw)[e];
     const float2* in = reinterpret_cast<const float2*>(a.in0);
     const float2* sink = reinterpret_cast<const float2*>(a.sink);
     // source of transform t: the input, or the sink past the batch end.  A
@@ -423,6 +431,11 @@ fft4096_mfma_stream_kernel(KArgs a) {
     // the prefetched registers are never copied
     float2 pa[4][4], pb[4][4];
     mfma4096_load(pa, src(f));
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+        const int e = tid + 256 * i;
+        if (e < kMfmaTwTotal) ltw[e] = twr[i];
+    }
     // FP16: the per-wave maxima of a transform are taken from the prefetch
     // registers at the end of the previous iteration, so the synchronisation
     // at the top of the next one publishes them and the transform needs none
