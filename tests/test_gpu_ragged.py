@@ -8,7 +8,7 @@ import pytest
 
 import oracle
 import zldsp_fft_amd as zf
-from zldsp_fft_amd import workloads
+from zldsp_fft_amd import autotune, workloads
 
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
@@ -108,7 +108,7 @@ def test_ragged_rejects_bad_descriptors():
     with pytest.raises(zf.ZlfftError) as e:
         zf.RaggedBatch([(zf.KIND_CFFT, 15, zf.FORWARD, zf.AOS, x, x)])
     assert e.value.code == 1
-    with pytest.raises(zf.ZlfftError) as e:  # 8 points per thread: 2^14 / 8 threads exceed one workgroup
+    with pytest.raises(zf.ZlfftError) as e:  # 8 points per thread: 2^14 / 8 threads exceed one workgroup (512)
         zf.RaggedBatch([(zf.KIND_CFFT, 14, zf.FORWARD, zf.AOS, x, x)], logr=[3] * 15)
     assert e.value.code == 5
     with pytest.raises(zf.ZlfftError) as e:
@@ -127,27 +127,28 @@ def _oracle(kind, direction, order, x):
 
 
 def test_ragged_points_per_thread_choices_agree():
-    """Both points-per-thread configurations the autotuner chooses between
-    give the same results (per transform, rel-L2 1e-6) on every order they
-    support, and match the FP64 oracle; the descriptor table is built in
-    bulk (RaggedBatch.from_table) over one buffer per variant."""
+    """Every points-per-thread configuration the autotuner chooses between
+    (8, 16, 32) gives the same results (per transform, rel-L2 1e-6) on every
+    order it supports, and matches the FP64 oracle; the descriptor table is
+    built in bulk (RaggedBatch.from_table) over one buffer per variant."""
     for order in range(3, 15):
         ut = workloads.UniformTable(order, 4 * len(workloads.VARIANTS), seed=order)
         res = {}
-        for logr in (3, 4):
-            if (logr == 4 and order < 4) or (logr == 3 and order > 13):
+        for logr in autotune.LOGRS:
+            if not autotune.legal(order, logr):
                 continue
-            table = [3 if n <= 3 else 4 for n in range(15)]
+            table = [autotune.default_logr(n) for n in range(15)]
             table[order] = logr  # also the CFFT order of the RFFT entries (real size 2^(order + 1))
             rb = zf.RaggedBatch.from_table(ut.table, keep=ut, logr=table)
             rb.run()
             torch.cuda.synchronize()
             res[logr] = [o.cpu().numpy().astype(np.float64) for o in ut.outputs()]
             rb.close()
-        if len(res) == 2:
-            for a, b in zip(res[3], res[4]):
+        first = min(res)
+        for lg in res:
+            for a, b in zip(res[lg], res[first]):
                 e = np.linalg.norm(a - b, axis=1) / np.linalg.norm(b, axis=1)
-                assert e.max() <= TOL, order
+                assert e.max() <= TOL, (order, lg)
         for v in range(len(ut.bufs)):
             (kind, direction, layout, o), x, got = ut.transform(v, 3)
             ref = _oracle(kind, direction, o, x)

@@ -1,14 +1,14 @@
-"""Autotuner: per CFFT order, the ragged kernel's points per thread (8 or 16,
-i.e. the radix of its passes, the threads per transform and the LDS tile per
-workgroup), measured on this GPU and written to zlfft_tune.txt next to
-libzlfft.so, where plans read it.
+"""Autotuner: per CFFT order, the ragged kernel's points per thread (8, 16 or
+32, i.e. the threads per transform, the transforms per workgroup and, at 32,
+whether the next item's input is prefetched), measured on this GPU and written
+to zlfft_tune.txt next to libzlfft.so, where plans read it.
 
     python -m zldsp_fft_amd.autotune [--out PATH] [--points 2**24]
 
 For every order, a batch of one size that cycles through every variant
 (CFFT / RFFT x forward / backward x AoS / SoA, sqr-mag) is timed with each
-legal choice; the two choices' outputs must agree (rel-L2 <= 1e-6) before one
-is kept.
+legal choice; the choices' outputs must agree (rel-L2 <= 1e-6) before one is
+kept.
 """
 from __future__ import annotations
 
@@ -23,8 +23,18 @@ import zldsp_fft_amd as zf  # noqa: E402
 from zldsp_fft_amd import workloads  # noqa: E402
 
 
-def _legal(order: int, logr: int) -> bool:
-    return (logr == 4 and order >= 4) or (logr == 3 and order >= 3 and (1 << (order - 3)) <= 1024)
+RAGGED_THREADS = 512   # threads per ragged workgroup
+LOGRS = (3, 4, 5)      # log2 points per thread the ragged kernel supports
+
+
+def legal(order: int, logr: int) -> bool:
+    """Whether the ragged kernel runs CFFT order `order` at 2^logr points per thread."""
+    return logr in LOGRS and order >= logr and (1 << (order - logr)) <= RAGGED_THREADS
+
+
+def default_logr(order: int) -> int:
+    """The library's choice without a tuning table."""
+    return 3 if order <= 3 else 4 if legal(order, 4) else 5
 
 
 def _rel(a, b):
@@ -38,14 +48,14 @@ def _rel(a, b):
 def tune(points: int, out_path: str, verbose: bool = True) -> dict:
     import torch
     results = {}
-    base = [3 if n <= 3 else 4 for n in range(15)]
+    base = [default_logr(n) for n in range(15)]
     for order in range(3, 15):
         count = max(len(workloads.VARIANTS), points >> order)
         ut = workloads.UniformTable(order, count, seed=order)
         times = {}
         outs = {}
-        for logr in (3, 4):
-            if not _legal(order, logr):
+        for logr in LOGRS:
+            if not legal(order, logr):
                 continue
             table = list(base)
             table[order] = logr
@@ -54,10 +64,11 @@ def tune(points: int, out_path: str, verbose: bool = True) -> dict:
             torch.cuda.synchronize()
             outs[logr] = [o.clone() for o in ut.outputs()]
             rb.close()
-        if len(outs) == 2:
-            worst = max(_rel(a, b) for a, b in zip(outs[3], outs[4]))
+        first = min(outs)
+        for lg in outs:
+            worst = max(_rel(a, b) for a, b in zip(outs[lg], outs[first]))
             if worst > 1e-6:
-                raise RuntimeError(f"order {order}: the two configurations disagree (rel-L2 {worst:.3e})")
+                raise RuntimeError(f"order {order}: configurations {first} and {lg} disagree (rel-L2 {worst:.3e})")
         del ut, outs
         torch.cuda.empty_cache()
         best = min(times, key=times.get)

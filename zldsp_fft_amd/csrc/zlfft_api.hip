@@ -64,7 +64,7 @@ void load_tune() {
         if (line[0] == '#') continue;
         if (sscanf(line, "%63s %d %d", key, &order, &value) != 3) continue;
         if (!strcmp(key, "ragged_logr") && order >= 0 && order <= zl::kRaggedMaxOrderTune &&
-            (value == 3 || value == 4))
+            value >= 3 && value <= 5)
             g_tune.ragged_logr[order] = value;
     }
     fclose(f);

@@ -2,18 +2,26 @@
 // layouts in one persistent launch.
 //
 // The host sorts the transforms into work items of one (CFFT order, variant):
-// up to TPW = RWG / T transforms (T = 2^n / 16 threads each, 16 points per
-// thread) that one workgroup of RWG threads handles together.  Items are
-// queued longest first; each persistent workgroup takes the next item from a
-// device counter until the queue is empty, so every workgroup ends when the
-// queue does.  Pointers come from each transform's descriptor.
+// up to TPW = RWG / T transforms (T = 2^n / R threads each, R = 8, 16 or 32
+// points per thread, chosen per order) that one workgroup of RWG threads
+// handles together.  It deals the
+// items to the workgroups longest first, each to the workgroup with the least
+// work so far (by bytes moved), so each workgroup owns a fixed run of items
+// and no queue is needed.  A workgroup streams through its run the way the
+// batched streaming kernel streams through a batch: the input of the next item
+// is loaded into registers while the current one is transformed and stored,
+// and the descriptors two items ahead.  The Stockham and RFFT twiddles come
+// from one small LDS table (W_{2^15}^e as two factors), so no global load sits
+// between the prefetch and its use.
 #pragma once
 
 #include "zlfft_device.hpp"
 
 namespace zl {
 
-constexpr int RWG = 1024;          // threads per ragged workgroup
+// threads per ragged workgroup: 8 waves, so each thread may hold 256 VGPRs
+// (the input of two items, 32 points each); the LDS holds one workgroup per CU
+constexpr int RWG = 512;
 constexpr int kRaggedMinOrder = 3;  // CFFT orders handled by the ragged kernel
 constexpr int kRaggedMaxOrder = 14;
 
@@ -41,22 +49,40 @@ struct RItem {
     int key;
 };
 
+// Twiddle table: W_{2^15}^e = exp(-2 pi i e / 2^15) for every e < 2^15 (2^15
+// is the real size of the largest RFFT, twice the largest CFFT) as two
+// factors, W^e = lo[e mod 256] * hi[e / 256].  lo is stored with one pad slot
+// per 16 entries: lanes reading entries 16 apart then hit different banks.
+constexpr int kMtBits = 15;
+constexpr int kMtLoBits = 8;
+constexpr int kMtLo = 1 << kMtLoBits;
+constexpr int kMtLoPadded = kMtLo + (kMtLo >> 4);
+constexpr int kMtHi = 1 << (kMtBits - kMtLoBits);
+constexpr int kMtTotal = kMtLoPadded + kMtHi;
+
+// bytes of the sink: inputs and outputs of lanes without a transform
+// (in0 / out0 at offset 0, in1 / out1 at kRaggedSinkHalf)
+constexpr long long kRaggedSinkHalf = 8ll * ((1 << kRaggedMaxOrder) + 64);
+constexpr long long kRaggedSinkBytes = 2 * kRaggedSinkHalf;
+constexpr int kRaggedWsink = 16;  // floats (one 64-byte line) per workgroup
+
 struct RArgs {
-    const RDesc* desc;
-    const RItem* items;
-    int nitems;
-    unsigned* counter;             // next item; zeroed before each launch
-    const float2* tw;              // Stockham twiddles of all orders, back to back
-    const float2* rtw;             // RFFT twiddles of all orders, back to back
-    long long tw_off[kRaggedMaxOrder + 1];
-    long long rtw_off[kRaggedMaxOrder + 1];
-    int logr[kRaggedMaxOrder + 1]; // points per thread (log2: 3 or 4) of each order
+    const RDesc* desc;    // descriptors; entry `sink` points at the sink
+    const RItem* items;   // work items, each workgroup's run contiguous
+    const int* runs;      // workgroup b: items [runs[4b], runs[4b+1]) at 32 points
+                          // per thread, then up to runs[4b+2] at 16, then up to
+                          // runs[4b+3] at 8
+    const float2* mt;     // twiddle table (kMtTotal entries)
+    float* wsink;         // stores that no lane needs: kRaggedWsink floats per workgroup
+    int sink;             // index of the sink descriptor
 };
 
-// Points per thread (log2) the ragged kernel supports for CFFT order n: 16
-// for every order, 8 where 2^n / 8 threads fit one workgroup.
+// Points per thread (log2: 3, 4 or 5) the ragged kernel supports for CFFT
+// order n: at most 2^n points, on at most RWG threads.
+constexpr int kRaggedMinLogr = 3;
+constexpr int kRaggedMaxLogr = 5;
 constexpr bool ragged_logr_ok(int n, int logr) {
-    return (logr == 4 && n >= 4) || (logr == 3 && n >= 3 && (1 << (n - 3)) <= RWG);
+    return logr >= kRaggedMinLogr && logr <= kRaggedMaxLogr && n >= logr && (1 << (n - logr)) <= RWG;
 }
 
 // LDS of a ragged workgroup: TPW = RWG / T transforms of 2^n points
@@ -65,7 +91,7 @@ constexpr bool ragged_logr_ok(int n, int logr) {
 constexpr int ragged_lds_elems() {
     int m = 0;
     for (int n = kRaggedMinOrder; n <= kRaggedMaxOrder; ++n)
-        for (int logr = 3; logr <= 4; ++logr) {
+        for (int logr = kRaggedMinLogr; logr <= kRaggedMaxLogr; ++logr) {
             if (!ragged_logr_ok(n, logr)) continue;
             const int N = 1 << n, T = N >> logr, tpw = T >= RWG ? 1 : RWG / T;
             const int ldsn = N + (N >> 4) + 1;
@@ -73,17 +99,55 @@ constexpr int ragged_lds_elems() {
         }
     return m;
 }
+static_assert(8ll * (ragged_lds_elems() + kMtTotal) <= 160 * 1024, "ragged LDS exceeds the CU's");
 
 __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 
+// t, opaque to the compiler: LDS addresses derived from it are recomputed at
+// each exchange instead of being computed once and held in registers for the
+// whole transform (R of them, with T known only at run time)
+__device__ __forceinline__ int rg_opaque(int t) {
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
+// Global-memory access through the descriptors' pointers.  They are loaded
+// from memory, so the compiler cannot tell they address global memory and
+// would emit flat loads, which count against the LDS wait counter as well:
+// every LDS wait of the transform would then wait for the prefetch.  The
+// casts make them global loads and stores.
+#define ZL_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ float2 rg_ld2(const void* p, int e) {  // complex e, non-temporal
+    const zl_f2v t = __builtin_nontemporal_load((const ZL_GLOBAL zl_f2v*)p + e);
+    return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ float rg_ld1(const void* p, int e) {
+    return __builtin_nontemporal_load((const ZL_GLOBAL float*)p + e);
+}
+__device__ __forceinline__ void rg_st2(void* p, int e, float2 v) {  // non-temporal
+    const zl_f2v t = {v.x, v.y};
+    __builtin_nontemporal_store(t, (ZL_GLOBAL zl_f2v*)p + e);
+}
+__device__ __forceinline__ void rg_st1(void* p, int e, float v) { __builtin_nontemporal_store(v, (ZL_GLOBAL float*)p + e); }
+__device__ __forceinline__ ZL_GLOBAL float* rg_gf(void* p) { return (ZL_GLOBAL float*)p; }
+__device__ __forceinline__ ZL_GLOBAL zl_f2v* rg_gf2(void* p) { return (ZL_GLOBAL zl_f2v*)p; }
+__device__ __forceinline__ void rg_put(ZL_GLOBAL zl_f2v* p, float2 v) { *p = zl_f2v{v.x, v.y}; }
+
+// W_{2^15}^e from the LDS table
+__device__ __forceinline__ float2 rg_tw(const float2* mt, int e) {
+    const int l = e & (kMtLo - 1);
+    return cmul(mt[pad16(l)], mt[kMtLoPadded + (e >> kMtLoBits)]);
+}
+
 // Stockham pass with radix r = 2^LR (compile time) and run-time width w = 2^lw
-// on this thread's R registers (nb = R / r butterflies i = t + T b); tw
-// points at the pass's twiddles W_{rw}^{mk} at [(m - 1) w + k].  Leaves the
-// outputs in v (v[b + q nb] = y_q of butterfly b).
+// on this thread's R registers (nb = R / r butterflies i = t + T b): twiddle
+// W_{r w}^{m k} = W_{2^15}^{(m k) << (15 - LR - lw)}.  Leaves the outputs in
+// v (v[b + q nb] = y_q of butterfly b).
 template <int R, int LR>
-__device__ __forceinline__ void rg_butterflies(float2 (&v)[R], int t, int T, int lw, const float2* __restrict__ tw) {
+__device__ __forceinline__ void rg_butterflies(float2 (&v)[R], int t, int T, int lw, const float2* mt) {
     constexpr int r = 1 << LR, nb = R / r;
     const int w = 1 << lw;
+    const int sh = kMtBits - LR - lw;
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
         float2 x[r];
@@ -92,11 +156,14 @@ __device__ __forceinline__ void rg_butterflies(float2 (&v)[R], int t, int T, int
         if (lw > 0) {
             const int k = (t + T * b) & (w - 1);
 #pragma unroll
-            for (int m = 1; m < r; ++m) x[m] = cmul(x[m], tw[(m - 1) * w + k]);
+            for (int m = 1; m < r; ++m) x[m] = cmul(x[m], rg_tw(mt, (m * k) << sh));
         }
         dft<r>(x);
 #pragma unroll
         for (int q = 0; q < r; ++q) v[b + q * nb] = x[q];
+        // one butterfly at a time: interleaved, the nb butterflies' temporaries
+        // would be live together on top of the R points and the prefetch
+        if (nb > 1) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -107,157 +174,237 @@ __device__ __forceinline__ void rg_exchange(float2 (&v)[R], float2* slds, int t,
     constexpr int r = 1 << LR, nb = R / r;
     const int w = 1 << lw;
     __syncthreads();
+    const int tw = rg_opaque(t);
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
-        const int i = t + T * b;
+        const int i = tw + T * b;
         const int base = ((i >> lw) << (lw + LR)) + (i & (w - 1));
 #pragma unroll
         for (int q = 0; q < r; ++q) slds[pad16(base + (q << lw))] = v[b + q * nb];
     }
     __syncthreads();
+    const int tr = rg_opaque(t);
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = slds[pad16(t + T * j)];
+    for (int j = 0; j < R; ++j) v[j] = slds[pad16(tr + T * j)];
 }
 
-// One work item of a ragged batch: `count` transforms of CFFT order n with
-// variant `key`, one per slot of T = 2^n / R threads.  One code path for
-// every order: the pass list (first pass radix 2^(n - LOGR(NP-1)), then radix
-// R) is walked at run time, so the persistent kernel holds one transform body
-// per point count R instead of one per order.  Same arithmetic as
-// fft_lds_kernel.
+// Kind / layout / direction of a variant key.
+struct RVariant {
+    int kind, lay, bwd;
+};
+__device__ __forceinline__ RVariant rg_variant(int key) {
+    RVariant r;
+    r.kind = key < 4 ? C2C : key < 6 ? R2C : key == 6 ? R2C_SQR : C2R;
+    r.lay = key < 4 ? (key >> 1) : key < 6 ? key - 4 : key == 6 ? AOS : key - 7;
+    r.bwd = key < 4 ? (key & 1) : r.kind == C2R;
+    return r;
+}
+
+// This lane's descriptor for item `it`: its slot's, or the sink's when the
+// slot is past the item's count.
 template <int LOGR>
-__device__ __forceinline__ void ragged_item(const RDesc* __restrict__ desc, const RItem& it,
-                                            const float2* __restrict__ tw, const float2* __restrict__ rtw,
-                                            float2* lds) {
+__device__ __forceinline__ RDesc rg_desc(const RArgs& a, const RItem& it, int tid) {
+    const int slot = tid >> (it.corder - LOGR);
+    return a.desc[slot < it.count ? it.first + slot : a.sink];
+}
+
+// Loads this lane's input of item `it` into p (element t + T j of its
+// transform, as complex: an RFFT's real input is read as 2^n pairs) and, for
+// C2R, bin 2^n into px.
+template <int LOGR>
+__device__ __forceinline__ void rg_prefetch(float2 (&p)[1 << LOGR], float2& px, const RItem& it, const RDesc& d,
+                                            int tid) {
+    constexpr int R = 1 << LOGR;
+    const int logt = it.corder - LOGR;
+    const int T = 1 << logt;
+    const int t = tid & (T - 1);
+    const RVariant vr = rg_variant(it.key);
+    const int lin = vr.kind == C2C || vr.kind == C2R ? vr.lay : AOS;
+    if (lin == AOS) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) p[j] = rg_ld2(d.in0, t + T * j);
+        if (vr.kind == C2R) px = rg_ld2(d.in0, 1 << it.corder);
+    } else {
+#pragma unroll
+        for (int j = 0; j < R; ++j) p[j] = make_float2(rg_ld1(d.in0, t + T * j), rg_ld1(d.in1, t + T * j));
+        if (vr.kind == C2R) px = make_float2(rg_ld1(d.in0, 1 << it.corder), rg_ld1(d.in1, 1 << it.corder));
+    }
+}
+
+// One work item: `count` transforms of CFFT order n with variant `key`, one
+// per slot of T = 2^n / R threads, from the input in v (and px).  One code
+// path for every order: the pass list (first pass radix 2^(n - LOGR(NP-1)),
+// then radix R) is walked at run time.  Same arithmetic as fft_lds_kernel
+// apart from the twiddles' rounding.  Lanes past the item's count run on the
+// sink's data and store to the sink, so every lane issues the same stores.
+template <int LOGR>
+__device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, const RItem& it, const RDesc& d,
+                                             float2* lds, const float2* mt, float* wsink, int tid) {
     constexpr int R = 1 << LOGR;
     const int n = it.corder;
     const int M = 1 << n;
     const int logt = n - LOGR;
     const int T = 1 << logt;
-    const int tid = threadIdx.x;
     const int slot = tid >> logt;
     const int t = tid & (T - 1);
-    const bool active = slot < it.count;
     const int ldsn = M + (M >> 4) + 1;
     float2* slds = lds + slot * ldsn;
-    const int key = it.key;
-    const int kind = key < 4 ? C2C : key < 6 ? R2C : key == 6 ? R2C_SQR : C2R;
-    const int lay = key < 4 ? (key >> 1) : key < 6 ? key - 4 : key == 6 ? AOS : key - 7;
-    const bool bwd = key < 4 ? (key & 1) != 0 : kind == C2R;
-    RDesc d{};
-    if (active) d = desc[it.first + slot];
+    const RVariant vr = rg_variant(it.key);
+    const int kind = vr.kind, lay = vr.lay;
 
-    float2 v[R];
     if (kind == C2R) {
-        if (active) {
-            if (t == 0) {
-                const float r0 = ld_c(d.in0, d.in1, lay, 0).x;
-                const float rm = ld_c(d.in0, d.in1, lay, M).x;
-                slds[0] = swapri(make_float2((r0 + rm) * 0.5f, (r0 - rm) * 0.5f));
-            }
-            for (int k = 1 + t; k <= M / 2; k += T) {
-                const float2 x1 = ld_c(d.in0, d.in1, lay, k);
-                const float2 x2 = ld_c(d.in0, d.in1, lay, M - k);
-                const float2 w = rtw[k - 1];
-                const float si = x1.y + x2.y, dr = x1.x - x2.x;
-                const float zr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(si, w.x, x2.x));
-                const float zi_m = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x2.y));
-                if (k != M - k) {
-                    const float zr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(-si, w.x, x1.x));
-                    const float zi_k = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x1.y));
-                    slds[pad16(k)] = make_float2(zi_k, zr_k);
-                }
-                slds[pad16(M - k)] = make_float2(zi_m, zr_m);
-            }
+        // pre-twiddle (reference src/common/zldsp_fft_common_execute.hpp:398-468)
+        // in place in LDS: thread t owns the bin pairs (k, M - k)
+        __syncthreads();
+        {
+            const int tw = rg_opaque(t);
+#pragma unroll
+            for (int j = 0; j < R; ++j) slds[pad16(tw + T * j)] = v[j];
         }
         __syncthreads();
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = slds[pad16(t + T * j)];
-    } else {
-        const int lin = kind == C2C ? lay : AOS;
-        if (active) {
-#pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const float2 x = ld_c(d.in0, d.in1, lin, t + T * j);
-                v[j] = bwd ? swapri(x) : x;
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < R; ++j) v[j] = make_float2(0.f, 0.f);
+        if (t == 0) {
+            const float r0 = slds[0].x;
+            const float rm = px.x;
+            slds[0] = swapri(make_float2((r0 + rm) * 0.5f, (r0 - rm) * 0.5f));
         }
+        const int rsh = kMtBits - 1 - n;  // W_{2M}^k = W_{2^15}^{k << rsh}
+#pragma unroll
+        for (int i = 0; i < R / 2; ++i) {
+            const int k = 1 + t + T * i;  // 1..M/2, each once
+            const float2 x1 = slds[pad16(k)];
+            const float2 x2 = slds[pad16(M - k)];
+            const float2 c = rg_tw(mt, k << rsh);
+            const float2 w = make_float2(0.5f * c.x, 0.5f - 0.5f * c.y);
+            const float si = x1.y + x2.y, dr = x1.x - x2.x;
+            const float zr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(si, w.x, x2.x));
+            const float zi_m = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x2.y));
+            if (k != M - k) {
+                const float zr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(-si, w.x, x1.x));
+                const float zi_k = __builtin_fmaf(dr, w.x, __builtin_fmaf(-si, w.y, x1.y));
+                slds[pad16(k)] = make_float2(zi_k, zr_k);
+            }
+            slds[pad16(M - k)] = make_float2(zi_m, zr_m);
+        }
+        __syncthreads();
+        const int tr = rg_opaque(t);
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = slds[pad16(tr + T * j)];
+    } else if (vr.bwd) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
     }
 
-    // passes: first radix 2^lr0, then radix R
-    const int np = (n + LOGR - 1) / LOGR;
-    const int lr0 = n - LOGR * (np - 1);
+    // passes: first radix 2^lr0, then radix 2^LP (LP = log2 of at most 16:
+    // a thread with 32 points runs two radix-16 butterflies per pass)
+    constexpr int LP = LOGR > 4 ? 4 : LOGR;
+    const int np = (n + LP - 1) / LP;
+    const int lr0 = n - LP * (np - 1);
     switch (lr0) {
-        case 1: rg_butterflies<R, 1>(v, t, T, 0, tw); break;
-        case 2: rg_butterflies<R, 2>(v, t, T, 0, tw); break;
-        case 3: rg_butterflies<R, 3>(v, t, T, 0, tw); break;
-        default: rg_butterflies<R, LOGR>(v, t, T, 0, tw); break;
+        case 1: rg_butterflies<R, 1>(v, t, T, 0, mt); break;
+        case 2: rg_butterflies<R, 2>(v, t, T, 0, mt); break;
+        case 3: rg_butterflies<R, 3>(v, t, T, 0, mt); break;
+        default: rg_butterflies<R, LP>(v, t, T, 0, mt); break;
     }
     int lw = lr0;
-    const float2* twp = tw;
     if (np > 1) {
         switch (lr0) {
             case 1: rg_exchange<R, 1>(v, slds, t, T, 0); break;
             case 2: rg_exchange<R, 2>(v, slds, t, T, 0); break;
             case 3: rg_exchange<R, 3>(v, slds, t, T, 0); break;
-            default: rg_exchange<R, LOGR>(v, slds, t, T, 0); break;
+            default: rg_exchange<R, LP>(v, slds, t, T, 0); break;
         }
     }
     for (int p = 1; p < np; ++p) {
-        rg_butterflies<R, LOGR>(v, t, T, lw, twp);
-        twp += (R - 1) << lw;
-        if (p + 1 < np) rg_exchange<R, LOGR>(v, slds, t, T, lw);
-        lw += LOGR;
+        rg_butterflies<R, LP>(v, t, T, lw, mt);
+        if (p + 1 < np) rg_exchange<R, LP>(v, slds, t, T, lw);
+        lw += LP;
     }
 
     if (kind == R2C || kind == R2C_SQR) {
+        // post-twiddle (reference src/common/zldsp_fft_common_execute.hpp:317-385,
+        // 480-552); every store is issued by every lane (the sink where a lane
+        // has no bin to write)
         __syncthreads();
+        {
+            const int tw = rg_opaque(t);
 #pragma unroll
-        for (int j = 0; j < R; ++j) slds[pad16(t + T * j)] = v[j];
+            for (int j = 0; j < R; ++j) slds[pad16(tw + T * j)] = v[j];
+        }
         __syncthreads();
-        if (active) {
-            if (t == 0) {
-                const float2 z0 = slds[0];
-                const float dc = z0.x + z0.y, ny = z0.x - z0.y;
-                if (kind == R2C) {
-                    st_c(d.out0, d.out1, lay, 0, make_float2(dc, 0.f));
-                    st_c(d.out0, d.out1, lay, M, make_float2(ny, 0.f));
-                } else {
-                    float* o = reinterpret_cast<float*>(d.out0);
-                    o[0] = dc * dc;
-                    o[M] = ny * ny;
-                }
-            }
-            for (int k = 1 + t; k <= M / 2; k += T) {
+        const float2 z0 = slds[0];
+        const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+        const int rsh = kMtBits - 1 - n;
+        // bins k and M - k for k = 1 + t + T i; put(k, k != M - k, X[k], X[M - k])
+        auto bins = [&](auto put) {
+#pragma unroll
+            for (int i = 0; i < R / 2; ++i) {
+                const int k = 1 + t + T * i;
                 const float2 z1 = slds[pad16(k)];
                 const float2 z2 = slds[pad16(M - k)];
-                const float2 w = rtw[k - 1];
+                const float2 c = rg_tw(mt, k << rsh);
+                const float2 w = make_float2(0.5f * c.x, 0.5f - 0.5f * c.y);
                 const float si = z1.y + z2.y, dr = z1.x - z2.x;
-                const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
-                const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
-                const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
-                const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
-                if (kind == R2C) {
-                    if (k != M - k) st_c(d.out0, d.out1, lay, k, make_float2(xr_k, xi_k));
-                    st_c(d.out0, d.out1, lay, M - k, make_float2(xr_m, xi_m));
-                } else {
-                    float* o = reinterpret_cast<float*>(d.out0);
-                    if (k != M - k) o[k] = __builtin_fmaf(xr_k, xr_k, xi_k * xi_k);
-                    o[M - k] = __builtin_fmaf(xr_m, xr_m, xi_m * xi_m);
-                }
+                const float2 xm = make_float2(__builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x)),
+                                              __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y)));
+                const float2 xk = make_float2(__builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x)),
+                                              __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y)));
+                put(k, k != M - k, xk, xm);
             }
+        };
+        // DC and Nyquist are thread 0's, and bin M/2 is written once, from the
+        // M - k formula (the reference's last store): the other lanes store
+        // those values to this workgroup's sink words
+        ZL_GLOBAL float* const ws = rg_gf(wsink);
+        if (kind == R2C && lay == AOS) {
+            ZL_GLOBAL zl_f2v* const o = rg_gf2(d.out0);
+            ZL_GLOBAL zl_f2v* const s2 = rg_gf2(wsink);
+            rg_put(t == 0 ? o : s2, make_float2(dc, 0.f));
+            rg_put(t == 0 ? o + M : s2 + 1, make_float2(ny, 0.f));
+            bins([&](int k, bool two, float2 xk, float2 xm) {
+                rg_put(two ? o + k : s2 + 2, xk);
+                rg_put(o + (M - k), xm);
+            });
+        } else if (kind == R2C) {
+            ZL_GLOBAL float* const o0 = rg_gf(d.out0);
+            ZL_GLOBAL float* const o1 = rg_gf(d.out1);
+            (t == 0 ? o0[0] : ws[0]) = dc;
+            (t == 0 ? o1[0] : ws[1]) = 0.f;
+            (t == 0 ? o0[M] : ws[2]) = ny;
+            (t == 0 ? o1[M] : ws[3]) = 0.f;
+            bins([&](int k, bool two, float2 xk, float2 xm) {
+                (two ? o0[k] : ws[4]) = xk.x;
+                (two ? o1[k] : ws[5]) = xk.y;
+                o0[M - k] = xm.x;
+                o1[M - k] = xm.y;
+            });
+        } else {
+            ZL_GLOBAL float* const o = rg_gf(d.out0);
+            (t == 0 ? o[0] : ws[0]) = dc * dc;
+            (t == 0 ? o[M] : ws[1]) = ny * ny;
+            bins([&](int k, bool two, float2 xk, float2 xm) {
+                (two ? o[k] : ws[2]) = __builtin_fmaf(xk.x, xk.x, xk.y * xk.y);
+                o[M - k] = __builtin_fmaf(xm.x, xm.x, xm.y * xm.y);
+            });
         }
     } else {
+        // C2C, and C2R's real output as 2^n pairs
         const int lout = kind == C2C ? lay : AOS;
-        if (active) {
+        if (vr.bwd) {
 #pragma unroll
-            for (int j = 0; j < R; ++j) st_c(d.out0, d.out1, lout, t + T * j, bwd ? swapri(v[j]) : v[j]);
+            for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
+        }
+        const int ts = rg_opaque(t);
+        if (lout == AOS) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) rg_st2(d.out0, ts + T * j, v[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                rg_st1(d.out0, ts + T * j, v[j].x);
+                rg_st1(d.out1, ts + T * j, v[j].y);
+            }
         }
     }
-    __syncthreads();  // the LDS is free for the next item
 }
 
 }  // namespace zl

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <queue>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -26,46 +28,38 @@ int hipf(hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hipf(e_, #call); \
     } while (0)
 
-// Stockham twiddles of a 2^n-point transform with a first pass of radix
-// 2^(n - logr(NP-1)) and radix-2^logr passes after it: pass p (width w) holds
-// W_{r w}^{m k} at [(m - 1) w + k]; FP64 on the GPU, rounded to FP32.
-__global__ void rg_gen_twiddles(float2* out, int n, int logr) {
-    const int np = (n + logr - 1) / logr, lr0 = n - logr * (np - 1);
-    long long off = 0;
-    int lw = lr0;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int p = 1; p < np; ++p) {
-        const long long cnt = ((1ll << logr) - 1) << lw;
-        if (idx < cnt) {
-            const long long w = 1ll << lw, m = idx / w + 1, k = idx % w, rw = 1ll << (lw + logr);
-            const long long e = (m * k) & (rw - 1);
-            double s, c;
-            sincospi(-2.0 * (double)e / (double)rw, &s, &c);
-            out[off + idx] = make_float2((float)c, (float)s);
-        }
-        off += cnt;
-        lw += logr;
+// The twiddle table of the ragged kernel: lo[e] = W_{2^15}^e (e < 256) at
+// pad16(e), then hi[a] = W_{2^15}^{256 a} (a < 128); FP64 on the GPU, rounded
+// to FP32.  Pad slots hold 0.
+__global__ void rg_gen_table(float2* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= zl::kMtTotal) return;
+    long long e = -1;
+    if (i < zl::kMtLoPadded) {
+        if ((i % 17) != 16) e = i - i / 17;  // pad16(e) = e + e / 16 = i
+    } else {
+        e = (long long)(i - zl::kMtLoPadded) << zl::kMtLoBits;
     }
-}
-
-// RFFT (wc, ws) for real size 2^(n+1), k = 1..2^n / 2
-// (reference src/common/zldsp_fft_common_init.hpp:374-386)
-__global__ void rg_gen_rfft_twiddles(float2* out, int n) {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int cnt = (1 << n) / 2;
-    if (idx >= cnt) return;
-    const double a = (double)(idx + 1) * (2.0 / (double)(2ll << n));
+    if (e < 0) {
+        out[i] = make_float2(0.f, 0.f);
+        return;
+    }
     double s, c;
-    sincospi(a, &s, &c);
-    out[idx] = make_float2((float)(0.5 * c), (float)(0.5 * s + 0.5));
+    sincospi(-2.0 * (double)e / (double)(1 << zl::kMtBits), &s, &c);
+    out[i] = make_float2((float)c, (float)s);
 }
 
-long long tw_count(int n, int logr) {
-    const int np = (n + logr - 1) / logr, lr0 = n - logr * (np - 1);
-    long long c = 0;
-    int lw = lr0;
-    for (int p = 1; p < np; ++p, lw += logr) c += ((1ll << logr) - 1) << lw;
-    return c;
+// points per thread (log2) of CFFT order n without a tuned value: 16, or 32
+// where 16 would need more than a workgroup's threads, 8 for order 3
+int default_logr(int n) { return n <= 3 ? 3 : zl::ragged_logr_ok(n, 4) ? 4 : 5; }
+
+// bytes one transform of the item's (corder, key) moves: read once, written once
+long long item_bytes(int corder, int key) {
+    const long long M = 1ll << corder;
+    if (key < 4) return 16 * M;                  // C2C
+    if (key < 6) return 8 * M + 8 * (M + 1);     // R2C
+    if (key == 6) return 8 * M + 4 * (M + 1);    // R2C_SQR
+    return 8 * (M + 1) + 8 * M;                  // C2R
 }
 
 }  // namespace
@@ -73,9 +67,9 @@ long long tw_count(int n, int logr) {
 struct zlfft_ragged_s {
     zl::RDesc* d_desc = nullptr;
     zl::RItem* d_items = nullptr;
-    unsigned* d_counter = nullptr;
-    float2* d_tw = nullptr;
-    float2* d_rtw = nullptr;
+    int* d_runs = nullptr;
+    float2* d_mt = nullptr;
+    char* d_sink = nullptr;
     int nitems = 0;
     hipStream_t stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -91,9 +85,9 @@ int zlfft_ragged_destroy(zlfft_ragged r) {
     if (r->graph) (void)hipGraphDestroy(r->graph);
     if (r->d_desc) (void)hipFree(r->d_desc);
     if (r->d_items) (void)hipFree(r->d_items);
-    if (r->d_counter) (void)hipFree(r->d_counter);
-    if (r->d_tw) (void)hipFree(r->d_tw);
-    if (r->d_rtw) (void)hipFree(r->d_rtw);
+    if (r->d_runs) (void)hipFree(r->d_runs);
+    if (r->d_mt) (void)hipFree(r->d_mt);
+    if (r->d_sink) (void)hipFree(r->d_sink);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
     return ZLFFT_OK;
@@ -107,10 +101,17 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
     using namespace zl;
     int lr[kRaggedMaxOrder + 1];
     for (int n = 0; n <= kRaggedMaxOrder; ++n) {
-        lr[n] = logr ? logr[n] : zl_tuned_ragged_logr(n);
-        if (n >= kRaggedMinOrder && !ragged_logr_ok(n, lr[n]))
-            return zl_fail(ZLFFT_ERR_INVALID_ARG, "points per thread 2^" + std::to_string(lr[n]) +
-                                                      " not available for order " + std::to_string(n));
+        if (logr) {
+            lr[n] = logr[n];
+            if (n >= kRaggedMinOrder && !ragged_logr_ok(n, lr[n]))
+                return zl_fail(ZLFFT_ERR_INVALID_ARG, "points per thread 2^" + std::to_string(lr[n]) +
+                                                          " not available for order " + std::to_string(n));
+        } else {
+            // a tuned value the kernel cannot run (a table from another
+            // build) falls back to the default
+            lr[n] = zl_tuned_ragged_logr(n);
+            if (n >= kRaggedMinOrder && !ragged_logr_ok(n, lr[n])) lr[n] = default_logr(n);
+        }
     }
     if (!out) return zl_fail(ZLFFT_ERR_NULL, "ragged out-pointer is NULL");
     *out = nullptr;
@@ -151,20 +152,67 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
         t[i].key = ragged_key(kind, s.layout, s.direction == ZLFFT_BACKWARD ? 1 : 0);
         t[i].d = RDesc{s.in0, s.in1, s.out0, s.out1};
     }
-    // longest transforms first, then by variant: items of one (corder, key)
+    // items of one (corder, key): descriptors sorted by order, then variant
     std::stable_sort(t.begin(), t.end(), [](const Tmp& a, const Tmp& b) {
         return a.corder != b.corder ? a.corder > b.corder : a.key < b.key;
     });
-    std::vector<RDesc> hd(count);
+    std::vector<RDesc> hd(count + 1);
     std::vector<RItem> hi;
     for (size_t i = 0; i < count; ++i) hd[i] = t[i].d;
     for (size_t i = 0; i < count;) {
         const int n = t[i].corder, key = t[i].key;
-        const int T = 1 << (n - lr[n]), tpw = T >= RWG ? 1 : RWG / T;
+        const int T = 1 << (n - lr[n]), tpw = RWG / T;
         size_t j = i;
         while (j < count && (int)(j - i) < tpw && t[j].corder == n && t[j].key == key) ++j;
         hi.push_back(RItem{(int)i, (int)(j - i), n, key});
         i = j;
+    }
+
+    int dev = 0, cus = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, zl_ragged_kernel(), RWG, 0);
+    if (e != hipSuccess) return hipf(e, "occupancy query");
+    const int grid = std::max(1, std::min(std::max(per_cu, 1) * cus, (int)hi.size()));
+
+    // Deal the items to the workgroups: largest first, each to the workgroup
+    // with the least bytes so far (plus a fixed cost per item).  A
+    // workgroup's run is ordered by points per thread (one loop each in the
+    // kernel), then largest first.
+    constexpr long long kItemCost = 16384;
+    std::vector<long long> cost(hi.size());
+    std::vector<int> order(hi.size());
+    for (size_t i = 0; i < hi.size(); ++i) {
+        cost[i] = hi[i].count * item_bytes(hi[i].corder, hi[i].key) + kItemCost;
+        order[i] = (int)i;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    using Load = std::pair<long long, int>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int g = 0; g < grid; ++g) heap.push({0, g});
+    std::vector<std::vector<int>> run(grid);
+    for (int i : order) {
+        Load l = heap.top();
+        heap.pop();
+        run[l.second].push_back(i);
+        l.first += cost[i];
+        heap.push(l);
+    }
+    std::vector<RItem> items;
+    items.reserve(hi.size());
+    std::vector<int> runs(4 * (size_t)grid);
+    for (int g = 0; g < grid; ++g) {
+        std::vector<int>& rg = run[g];
+        std::stable_sort(rg.begin(), rg.end(), [&](int x, int y) {
+            const int lx = lr[hi[x].corder], ly = lr[hi[y].corder];
+            return lx != ly ? lx > ly : cost[x] > cost[y];
+        });
+        size_t k = 0;
+        runs[4 * g] = (int)items.size();
+        for (int L = kRaggedMaxLogr; L >= kRaggedMinLogr; --L) {
+            while (k < rg.size() && lr[hi[rg[k]].corder] == L) items.push_back(hi[rg[k++]]);
+            runs[4 * g + 1 + (kRaggedMaxLogr - L)] = (int)items.size();
+        }
     }
 
     zlfft_ragged r = new (std::nothrow) zlfft_ragged_s();
@@ -173,60 +221,43 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
         zlfft_ragged_destroy(r);
         return rc;
     };
-    r->nitems = (int)hi.size();
-    hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+    r->nitems = (int)items.size();
+    e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hipf(e, "hipStreamCreateWithFlags"));
-    RArgs a{};
-    long long ntw = 0, nrtw = 0;
-    for (int n = kRaggedMinOrder; n <= kRaggedMaxOrder; ++n) {
-        a.tw_off[n] = ntw;
-        a.rtw_off[n] = nrtw;
-        a.logr[n] = lr[n];
-        ntw += tw_count(n, lr[n]);
-        nrtw += std::max(1ll, (1ll << n) / 2);
-    }
-    if (hipMalloc(&r->d_desc, sizeof(RDesc) * std::max<size_t>(count, 1)) != hipSuccess ||
-        hipMalloc(&r->d_items, sizeof(RItem) * std::max<size_t>(hi.size(), 1)) != hipSuccess ||
-        hipMalloc(&r->d_counter, sizeof(unsigned)) != hipSuccess ||
-        hipMalloc(&r->d_tw, sizeof(float2) * (size_t)ntw) != hipSuccess ||
-        hipMalloc(&r->d_rtw, sizeof(float2) * (size_t)nrtw) != hipSuccess)
+    if (hipMalloc(&r->d_desc, sizeof(RDesc) * hd.size()) != hipSuccess ||
+        hipMalloc(&r->d_items, sizeof(RItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
+        hipMalloc(&r->d_runs, sizeof(int) * runs.size()) != hipSuccess ||
+        hipMalloc(&r->d_mt, sizeof(float2) * kMtTotal) != hipSuccess ||
+        hipMalloc(&r->d_sink, kRaggedSinkBytes + 4ll * kRaggedWsink * grid) != hipSuccess)
         return cleanup(zl_fail(ZLFFT_ERR_ALLOC, "ragged hipMalloc failed"));
-    if (count) {
-        e = hipMemcpyAsync(r->d_desc, hd.data(), sizeof(RDesc) * count, hipMemcpyHostToDevice, r->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(r->d_items, hi.data(), sizeof(RItem) * hi.size(), hipMemcpyHostToDevice, r->stream);
-        if (e != hipSuccess) return cleanup(hipf(e, "hipMemcpyAsync"));
-    }
-    for (int n = kRaggedMinOrder; n <= kRaggedMaxOrder; ++n) {
-        const long long c = tw_count(n, lr[n]);
-        if (c > 0)
-            rg_gen_twiddles<<<(unsigned)((c + 255) / 256), 256, 0, r->stream>>>(r->d_tw + a.tw_off[n], n, lr[n]);
-        rg_gen_rfft_twiddles<<<((1 << n) / 2 + 255) / 256, 256, 0, r->stream>>>(r->d_rtw + a.rtw_off[n], n);
-    }
+    // the sink descriptor: lanes without a transform read and write there
+    hd[count] = RDesc{r->d_sink, r->d_sink + kRaggedSinkHalf, r->d_sink, r->d_sink + kRaggedSinkHalf};
+    e = hipMemsetAsync(r->d_sink, 0, kRaggedSinkBytes + 4ll * kRaggedWsink * grid, r->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r->d_desc, hd.data(), sizeof(RDesc) * hd.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess && !items.empty())
+        e = hipMemcpyAsync(r->d_items, items.data(), sizeof(RItem) * items.size(), hipMemcpyHostToDevice, r->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(r->d_runs, runs.data(), sizeof(int) * runs.size(), hipMemcpyHostToDevice, r->stream);
+    if (e != hipSuccess) return cleanup(hipf(e, "ragged setup copies"));
+    rg_gen_table<<<(kMtTotal + 255) / 256, 256, 0, r->stream>>>(r->d_mt);
     e = hipStreamSynchronize(r->stream);
     if (e != hipSuccess) return cleanup(hipf(e, "ragged setup"));
 
+    RArgs a{};
     a.desc = r->d_desc;
     a.items = r->d_items;
-    a.nitems = r->nitems;
-    a.counter = r->d_counter;
-    a.tw = r->d_tw;
-    a.rtw = r->d_rtw;
-    int dev = 0, cus = 0, per_cu = 0;
-    e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, zl_ragged_kernel(), RWG, 0);
-    if (e != hipSuccess) return cleanup(hipf(e, "occupancy query"));
-    const int grid = std::max(1, std::min(per_cu * cus, r->nitems));
+    a.runs = r->d_runs;
+    a.mt = r->d_mt;
+    a.wsink = reinterpret_cast<float*>(r->d_sink + kRaggedSinkBytes);
+    a.sink = (int)count;
 
-    // capture: reset the queue counter (a kernel, not a memset node), then the
-    // persistent kernel
+    // capture: the persistent kernel (each workgroup runs its own items, so
+    // there is no queue to reset between launches)
     e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return cleanup(hipf(e, "hipStreamBeginCapture"));
-    void* rp[] = {&r->d_counter};
-    e = hipLaunchKernel(zl_ragged_reset_kernel(), dim3(1), dim3(64), rp, 0, r->stream);
     void* kp[] = {&a};
-    if (e == hipSuccess) e = hipLaunchKernel(zl_ragged_kernel(), dim3(grid), dim3(RWG), kp, 0, r->stream);
+    e = hipLaunchKernel(zl_ragged_kernel(), dim3(grid), dim3(RWG), kp, 0, r->stream);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(r->stream, &g);
     r->graph = g;

@@ -1,34 +1,104 @@
-// The ragged-batch kernels (one translation unit: the kernel dispatches over
-// every order and kind).
+// The ragged-batch kernel (one translation unit: it dispatches over every
+// order and kind at run time).
 #include "zlfft_ragged.hpp"
 
 namespace zl {
 
-__global__ void __launch_bounds__(RWG) fft_ragged_kernel(RArgs a) {
-    __shared__ float2 lds[ragged_lds_elems()];
-    __shared__ int next;
-    for (;;) {
-        if (threadIdx.x == 0) next = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        const int i = next;
-        __syncthreads();  // everyone has read `next` before it changes
-        if (i >= a.nitems) break;
-        const RItem it = a.items[i];
-        const float2* tw = a.tw + a.tw_off[it.corder];
-        const float2* rtw = a.rtw + a.rtw_off[it.corder];
-        if (a.logr[it.corder] == 3)
-            ragged_item<3>(a.desc, it, tw, rtw, lds);
-        else
-            ragged_item<4>(a.desc, it, tw, rtw, lds);
+// A workgroup's run of items at 2^LOGR points per thread, items [beg, end).
+// With 8 or 16 points per thread, step i holds item i's input in p (loaded
+// during step i - 1), loads item i + 2's descriptors and item i + 1's input,
+// then transforms and stores item i.  The first step is peeled off the loop so
+// the loop header is reached only from a copy of the loop body: both ways in
+// then have the same loads and stores in flight, and the header's wait covers
+// the input loads only.  With 32 points per thread (CFFT order 14) the
+// transform alone takes most of the 256 VGPRs, so an item's input is loaded
+// at its own step, with its descriptors loaded one step ahead.
+template <int LOGR>
+__device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2* lds, float2* mt,
+                                       const float2 (&mt_reg)[2], bool fill_mt) {
+    constexpr int R = 1 << LOGR;
+    constexpr bool prefetch = LOGR <= 4;
+    const int tid = threadIdx.x;
+    float* const ws = a.wsink + kRaggedWsink * blockIdx.x;
+    // items past the run: no transform (every lane on the sink)
+    auto item = [&](int i) {
+        const RItem r = a.items[i < end ? i : end - 1];
+        return i < end ? r : RItem{0, 0, LOGR, 0};
+    };
+    auto fill = [&]() {
+        if (fill_mt) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int e = tid + RWG * i;
+                if (e < kMtTotal) mt[e] = mt_reg[i];
+            }
+            __syncthreads();
+        }
+    };
+    RItem ic = item(beg);
+    RItem in = item(beg + 1);
+    RDesc dc = rg_desc<LOGR>(a, ic, tid);
+    RDesc dn = rg_desc<LOGR>(a, in, tid);
+    if constexpr (prefetch) {
+        float2 p[R], px = make_float2(0.f, 0.f);
+        rg_prefetch<LOGR>(p, px, ic, dc, tid);
+        fill();
+        auto step = [&](int i) {
+            float2 v[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[j] = p[j];
+            const float2 vx = px;
+            const RItem i2 = item(i + 2);
+            const RDesc d2 = rg_desc<LOGR>(a, i2, tid);
+            rg_prefetch<LOGR>(p, px, in, dn, tid);
+            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, ws, tid);
+            ic = in;
+            dc = dn;
+            in = i2;
+            dn = d2;
+        };
+        step(beg);
+        for (int i = beg + 1; i < end; ++i) step(i);
+    } else {
+        fill();
+        for (int i = beg; i < end; ++i) {
+            float2 v[R], vx = make_float2(0.f, 0.f);
+            rg_prefetch<LOGR>(v, vx, ic, dc, tid);
+            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, ws, tid);
+            ic = in;
+            dc = dn;
+            in = item(i + 2);
+            dn = rg_desc<LOGR>(a, in, tid);
+        }
     }
 }
 
-// zeroes the queue counter (a kernel node, so the graph holds no memset node)
-__global__ void ragged_reset_kernel(unsigned* counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+__global__ void __launch_bounds__(RWG) fft_ragged_kernel(RArgs a) {
+    __shared__ float2 lds[ragged_lds_elems()];
+    __shared__ float2 mt[kMtTotal];
+    const int b = blockIdx.x;
+    const int r0 = a.runs[4 * b], r1 = a.runs[4 * b + 1], r2 = a.runs[4 * b + 2], r3 = a.runs[4 * b + 3];
+    const int tid = threadIdx.x;
+    // the twiddle table (kMtTotal <= 2 RWG entries) is loaded here and
+    // written to LDS after the first run's first prefetch is issued
+    float2 mt_reg[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int e = tid + RWG * i;
+        mt_reg[i] = e < kMtTotal ? a.mt[e] : make_float2(0.f, 0.f);
+    }
+    bool fill = true;
+    if (r0 < r1) {
+        rg_run<5>(a, r0, r1, lds, mt, mt_reg, fill);
+        fill = false;
+    }
+    if (r1 < r2) {
+        rg_run<4>(a, r1, r2, lds, mt, mt_reg, fill);
+        fill = false;
+    }
+    if (r2 < r3) rg_run<3>(a, r2, r3, lds, mt, mt_reg, fill);
 }
 
 }  // namespace zl
 
 void* zl_ragged_kernel() { return reinterpret_cast<void*>(zl::fft_ragged_kernel); }
-void* zl_ragged_reset_kernel() { return reinterpret_cast<void*>(zl::ragged_reset_kernel); }

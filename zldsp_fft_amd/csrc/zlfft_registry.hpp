@@ -176,9 +176,8 @@ MacroEntry make_macro_entry() {
 // per workgroup, or the persistent streaming form), or null
 void* zl_mfma_kernel(int prec, int backward, int streaming);
 
-// ragged-batch kernels (zlfft_ragged_k.hip)
+// ragged-batch kernel (zlfft_ragged_k.hip)
 void* zl_ragged_kernel();
-void* zl_ragged_reset_kernel();
 
 #define ZL_DECLARE_MACRO(n)                            \
     void zl_fill_macro_##n(zl::MacroTable& tab);       \
