@@ -23,13 +23,13 @@ import zldsp_fft_amd as zf  # noqa: E402
 from zldsp_fft_amd import workloads  # noqa: E402
 
 
-RAGGED_THREADS = 512   # threads per ragged workgroup
 LOGRS = (3, 4, 5)      # log2 points per thread the ragged kernel supports
 
 
 def legal(order: int, logr: int) -> bool:
-    """Whether the ragged kernel runs CFFT order `order` at 2^logr points per thread."""
-    return logr in LOGRS and order >= logr and (1 << (order - logr)) <= RAGGED_THREADS
+    """Whether the ragged kernel runs CFFT order `order` at 2^logr points per
+    thread: 8 for orders 3-7, 16 for 4-13, 32 for 8-14."""
+    return (logr == 3 and 3 <= order <= 7) or (logr == 4 and 4 <= order <= 13) or (logr == 5 and 8 <= order <= 14)
 
 
 def default_logr(order: int) -> int:

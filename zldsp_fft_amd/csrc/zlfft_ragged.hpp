@@ -77,12 +77,14 @@ struct RArgs {
     int sink;             // index of the sink descriptor
 };
 
-// Points per thread (log2: 3, 4 or 5) the ragged kernel supports for CFFT
-// order n: at most 2^n points, on at most RWG threads.
+// Points per thread (log2) the ragged kernel supports for CFFT order n: 8
+// for orders 3-7, 16 for orders 4-13, 32 for orders 8-14 (at most RWG
+// threads per transform; each pair is a pass chain of its own in the kernel,
+// so the small-order / large-point pairs no order is tuned to are left out)
 constexpr int kRaggedMinLogr = 3;
 constexpr int kRaggedMaxLogr = 5;
 constexpr bool ragged_logr_ok(int n, int logr) {
-    return logr >= kRaggedMinLogr && logr <= kRaggedMaxLogr && n >= logr && (1 << (n - logr)) <= RWG;
+    return (logr == 3 && n >= 3 && n <= 7) || (logr == 4 && n >= 4 && n <= 13) || (logr == 5 && n >= 8 && n <= 14);
 }
 
 // LDS of a ragged workgroup: TPW = RWG / T transforms of 2^n points
@@ -139,53 +141,77 @@ __device__ __forceinline__ float2 rg_tw(const float2* mt, int e) {
     return cmul(mt[pad16(l)], mt[kMtLoPadded + (e >> kMtLoBits)]);
 }
 
-// Stockham pass with radix r = 2^LR (compile time) and run-time width w = 2^lw
-// on this thread's R registers (nb = R / r butterflies i = t + T b): twiddle
-// W_{r w}^{m k} = W_{2^15}^{(m k) << (15 - LR - lw)}.  Leaves the outputs in
-// v (v[b + q nb] = y_q of butterfly b).
-template <int R, int LR>
-__device__ __forceinline__ void rg_butterflies(float2 (&v)[R], int t, int T, int lw, const float2* mt) {
-    constexpr int r = 1 << LR, nb = R / r;
-    const int w = 1 << lw;
-    const int sh = kMtBits - LR - lw;
+// Pass geometry of CFFT order NL at 2^LOGR points per thread, all compile
+// time: T = 2^NL / R threads per transform; a first pass of radix 2^lr0, then
+// radix 2^LP passes (LP = log2 of at most 16: a thread with 32 points runs
+// two radix-16 butterflies per pass).  Each (order, points) pair the kernel
+// runs has its own pass chain, so LDS offsets and twiddle exponents are
+// constants or shifts of t, as in the batched kernels.
+template <int NL, int LOGR>
+struct RGeo {
+    static constexpr int R = 1 << LOGR;
+    static constexpr int T = 1 << (NL - LOGR);
+    static constexpr int LP = LOGR > 4 ? 4 : LOGR;
+    static constexpr int NP = (NL + LP - 1) / LP;
+    static constexpr int LR0 = NL - LP * (NP - 1);
+    static constexpr int lr(int p) { return p == 0 ? LR0 : LP; }
+    static constexpr int lw(int p) { return p == 0 ? 0 : LR0 + LP * (p - 1); }
+};
+
+// Pass P of geometry G on this thread's R registers (nb = R / r butterflies
+// i = t + T b), twiddle W_{r w}^{m k} = W_{2^15}^{(m k) << (15 - lr - lw)},
+// then the LDS exchange to the next pass's inputs and the next pass.
+template <class G, int P>
+__device__ __forceinline__ void rgc_pass(float2 (&v)[G::R], float2* slds, const float2* mt, int t) {
+    constexpr int LR = G::lr(P), r = 1 << LR, nb = G::R / r, lw = G::lw(P), w = 1 << lw;
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
         float2 x[r];
 #pragma unroll
         for (int m = 0; m < r; ++m) x[m] = v[b + m * nb];
-        if (lw > 0) {
-            const int k = (t + T * b) & (w - 1);
+        if constexpr (P > 0) {
+            constexpr int sh = kMtBits - LR - lw;
+            const int k = (t + G::T * b) & (w - 1);
 #pragma unroll
             for (int m = 1; m < r; ++m) x[m] = cmul(x[m], rg_tw(mt, (m * k) << sh));
         }
         dft<r>(x);
 #pragma unroll
         for (int q = 0; q < r; ++q) v[b + q * nb] = x[q];
-        // one butterfly at a time: interleaved, the nb butterflies' temporaries
-        // would be live together on top of the R points and the prefetch
-        if (nb > 1) __builtin_amdgcn_sched_barrier(0);
+        // one butterfly at a time: interleaved, the nb butterflies'
+        // temporaries would be live together on top of the points and the
+        // prefetch
+        if constexpr (nb > 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (P + 1 < G::NP) {
+        __syncthreads();
+#pragma unroll
+        for (int b = 0; b < nb; ++b) {
+            const int i = t + G::T * b;
+            const int base = ((i >> lw) << (lw + LR)) + (i & (w - 1));
+#pragma unroll
+            for (int q = 0; q < r; ++q) slds[pad16(base + (q << lw))] = v[b + q * nb];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < G::R; ++j) v[j] = slds[pad16(t + G::T * j)];
+        rgc_pass<G, P + 1>(v, slds, mt, t);
     }
 }
 
-// LDS exchange after a pass of radix 2^lr and width 2^lw: outputs to their
-// Stockham positions, then reload v[j] = element t + T j.
-template <int R, int LR>
-__device__ __forceinline__ void rg_exchange(float2 (&v)[R], float2* slds, int t, int T, int lw) {
-    constexpr int r = 1 << LR, nb = R / r;
-    const int w = 1 << lw;
-    __syncthreads();
-    const int tw = rg_opaque(t);
-#pragma unroll
-    for (int b = 0; b < nb; ++b) {
-        const int i = tw + T * b;
-        const int base = ((i >> lw) << (lw + LR)) + (i & (w - 1));
-#pragma unroll
-        for (int q = 0; q < r; ++q) slds[pad16(base + (q << lw))] = v[b + q * nb];
+// the pass chain of order n (one compare per order the kernel runs at 2^LOGR
+// points per thread)
+template <int LOGR, int NL = kRaggedMinOrder>
+__device__ __forceinline__ void rgc_passes(int n, float2 (&v)[1 << LOGR], float2* slds, const float2* mt, int t) {
+    if constexpr (NL <= kRaggedMaxOrder) {
+        if constexpr (ragged_logr_ok(NL, LOGR)) {
+            if (n == NL) {
+                rgc_pass<RGeo<NL, LOGR>, 0>(v, slds, mt, t);
+                return;
+            }
+        }
+        rgc_passes<LOGR, NL + 1>(n, v, slds, mt, t);
     }
-    __syncthreads();
-    const int tr = rg_opaque(t);
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = slds[pad16(tr + T * j)];
 }
 
 // Kind / layout / direction of a variant key.
@@ -294,31 +320,7 @@ __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, 
         for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
     }
 
-    // passes: first radix 2^lr0, then radix 2^LP (LP = log2 of at most 16:
-    // a thread with 32 points runs two radix-16 butterflies per pass)
-    constexpr int LP = LOGR > 4 ? 4 : LOGR;
-    const int np = (n + LP - 1) / LP;
-    const int lr0 = n - LP * (np - 1);
-    switch (lr0) {
-        case 1: rg_butterflies<R, 1>(v, t, T, 0, mt); break;
-        case 2: rg_butterflies<R, 2>(v, t, T, 0, mt); break;
-        case 3: rg_butterflies<R, 3>(v, t, T, 0, mt); break;
-        default: rg_butterflies<R, LP>(v, t, T, 0, mt); break;
-    }
-    int lw = lr0;
-    if (np > 1) {
-        switch (lr0) {
-            case 1: rg_exchange<R, 1>(v, slds, t, T, 0); break;
-            case 2: rg_exchange<R, 2>(v, slds, t, T, 0); break;
-            case 3: rg_exchange<R, 3>(v, slds, t, T, 0); break;
-            default: rg_exchange<R, LP>(v, slds, t, T, 0); break;
-        }
-    }
-    for (int p = 1; p < np; ++p) {
-        rg_butterflies<R, LP>(v, t, T, lw, mt);
-        if (p + 1 < np) rg_exchange<R, LP>(v, slds, t, T, lw);
-        lw += LP;
-    }
+    rgc_passes<LOGR>(n, v, slds, mt, t);
 
     if (kind == R2C || kind == R2C_SQR) {
         // post-twiddle (reference src/common/zldsp_fft_common_execute.hpp:317-385,

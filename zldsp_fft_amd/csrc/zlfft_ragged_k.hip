@@ -43,7 +43,7 @@ __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2*
         float2 p[R], px = make_float2(0.f, 0.f);
         rg_prefetch<LOGR>(p, px, ic, dc, tid);
         fill();
-        auto step = [&](int i) {
+        auto step = [&](int i) __attribute__((always_inline)) {
             float2 v[R];
 #pragma unroll
             for (int j = 0; j < R; ++j) v[j] = p[j];
