@@ -182,7 +182,7 @@ typedef struct zlfft_ragged_s *zlfft_ragged;
 /* Builds the work queue, twiddles and the captured graph (no counterpart in
  * the reference, whose calls are one transform each). */
 int zlfft_ragged_create(zlfft_ragged *ragged, const zlfft_ragged_desc *descs, size_t count);
-/* As zlfft_ragged_create, with the points per thread (log2: 3 or 4) of each
+/* As zlfft_ragged_create, with the points per thread (log2: 3, 4 or 5) of each
  * CFFT order 0..14 given in logr[15] instead of the tuned table
  * (zlfft_tune.txt next to the library, or $ZLFFT_TUNE_FILE); for the
  * autotuner. */
