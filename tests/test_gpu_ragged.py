@@ -100,6 +100,58 @@ def test_ragged_sqr_mag_and_small_orders():
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 2 * TOL, (kind, order)
 
 
+def test_ragged_single_empty_and_lopsided():
+    """Edge cases of the per-workgroup runs: one transform (one workgroup),
+    an empty set (a no-op), and one order-14 CFFT among 3000 order-3 ones
+    (most workgroups run only tiny items, one runs the large one), all
+    layouts of the large one; each output against the FP64 oracle, and the
+    large one also against the plain path."""
+    g = torch.Generator(device="cuda").manual_seed(7)
+
+    def cplx(n):
+        return torch.view_as_complex((torch.rand((n, 2), generator=g, device="cuda") * 2 - 1).contiguous())
+
+    x = cplx(1 << 14)
+    y = torch.empty_like(x)
+    rb = zf.RaggedBatch([(zf.KIND_CFFT, 14, zf.FORWARD, zf.AOS, x, y)])
+    rb.run()
+    torch.cuda.synchronize()
+    ref = oracle.cfft(x.cpu().numpy().astype(np.complex128), True, "f64")
+    assert np.linalg.norm(y.cpu().numpy() - ref) / np.linalg.norm(ref) <= TOL
+    rb.close()
+
+    empty = zf.RaggedBatch([])
+    empty.run()
+    torch.cuda.synchronize()
+    empty.close()
+
+    for layout in (zf.AOS, zf.SOA):
+        big = cplx(1 << 14)
+        if layout == zf.SOA:
+            bin_ = (big.real.contiguous(), big.imag.contiguous())
+            bout = (torch.empty(1 << 14, device="cuda"), torch.empty(1 << 14, device="cuda"))
+        else:
+            bin_, bout = big, torch.empty_like(big)
+        descs = [(zf.KIND_CFFT, 14, zf.BACKWARD, layout, bin_, bout)]
+        for _ in range(3000):
+            t = cplx(8)
+            descs.append((zf.KIND_CFFT, 3, zf.FORWARD, zf.AOS, t, torch.empty_like(t)))
+        rb = zf.RaggedBatch(descs)
+        rb.run()
+        torch.cuda.synchronize()
+        got = as_np(descs[0][5])
+        ref = oracle.cfft(big.cpu().numpy().astype(np.complex128), False, "f64")
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= TOL, layout
+        pl = as_np(plain(descs[0], {}))
+        assert np.linalg.norm(got - pl) / np.linalg.norm(pl) <= TOL, layout
+        xs = np.stack([d[4].cpu().numpy() for d in descs[1:]]).astype(np.complex128)
+        ys = np.stack([d[5].cpu().numpy() for d in descs[1:]])
+        refs = np.fft.fft(xs, axis=1)  # order 3 forward: the 8-point DFT
+        e = np.linalg.norm(ys - refs, axis=1) / np.linalg.norm(refs, axis=1)
+        assert e.max() <= TOL, layout
+        rb.close()
+
+
 def test_ragged_rejects_bad_descriptors():
     x = torch.zeros(64, dtype=torch.complex64, device="cuda")
     with pytest.raises(zf.ZlfftError) as e:

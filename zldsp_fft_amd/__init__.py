@@ -422,8 +422,8 @@ class RaggedBatch:
     """
 
     def __init__(self, descs, logr=None):
-        """``logr``: optional points per thread (log2, 3 or 4) for CFFT orders
-        0..14 instead of the tuned table."""
+        """``logr``: optional points per thread (log2: 3, 4 or 5) for CFFT
+        orders 0..14 instead of the tuned table."""
         table = np.zeros(len(descs), dtype=RAGGED_DESC_DTYPE)
         for i, (kind, order, direction, layout, inb, outb) in enumerate(descs):
             def ptrs(b):
