@@ -143,15 +143,18 @@ __device__ __forceinline__ float2 rg_tw(const float2* mt, int e) {
 
 // Pass geometry of CFFT order NL at 2^LOGR points per thread, all compile
 // time: T = 2^NL / R threads per transform; a first pass of radix 2^lr0, then
-// radix 2^LP passes (LP = log2 of at most 16: a thread with 32 points runs
-// two radix-16 butterflies per pass).  Each (order, points) pair the kernel
-// runs has its own pass chain, so LDS offsets and twiddle exponents are
-// constants or shifts of t, as in the batched kernels.
+// radix 2^LP passes.  At 32 points per thread LP is 5 where radix-32 passes
+// save a pass over radix 16 (orders 9, 10, 13, 14), else 4: the thread then
+// runs two radix-16 butterflies per pass (measured per order, 2^24 points:
+// order 14 0.093 vs 0.100 ms, orders 9-10 0.083 vs 0.091, order 12 0.101 vs
+// 0.095 with radix 32 and 16).  Each (order, points) pair the kernel runs
+// has its own pass chain, so LDS offsets and twiddle exponents are constants
+// or shifts of t, as in the batched kernels.
 template <int NL, int LOGR>
 struct RGeo {
     static constexpr int R = 1 << LOGR;
     static constexpr int T = 1 << (NL - LOGR);
-    static constexpr int LP = LOGR > 4 ? 4 : LOGR;
+    static constexpr int LP = LOGR <= 4 ? LOGR : (NL + 4) / 5 < (NL + 3) / 4 ? 5 : 4;
     static constexpr int NP = (NL + LP - 1) / LP;
     static constexpr int LR0 = NL - LP * (NP - 1);
     static constexpr int lr(int p) { return p == 0 ? LR0 : LP; }
