@@ -871,62 +871,97 @@ fft_stream_kernel(KArgs a) {
 #pragma unroll
             for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, LOUT, eo + t + G::T * j, bwd ? swapri(v[j]) : v[j]);
         } else {
-            // post-twiddle epilogue, identical to fft_lds_kernel's.  A frame's
-            // M + 1 outputs start off a cache-line boundary; with cached_out
-            // the stores go through the cache, which merges the lines that
-            // neighbouring frames share (the host picks it by batch size).
-            const bool cached = a.cached_out != 0;
-            auto rst_c = [&](void* o0, void* o1, int lay, long long e, float2 v) {
-                if (cached) st_c(o0, o1, lay, e, v); else st_c_nt(o0, o1, lay, e, v);
-            };
-            auto rst = [&](float* p, float v) {
-                if (cached) *p = v; else st_nt(p, v);
-            };
+            // post-twiddle epilogue, the arithmetic of fft_lds_kernel's.  A
+            // frame's M + 1 outputs start off a cache-line boundary; with
+            // cached_out the stores go through the cache, which merges the
+            // lines that neighbouring frames share (the host picks it by batch
+            // size).  Every lane issues the same stores in straight-line code:
+            // bins k and M - k for k = 1 + t + T i, i < R / 2 (each k in
+            // 1..M/2 once), and DC and Nyquist, which every lane of the frame
+            // computes from Z[0] and stores alike.  Bin M/2 gets the M - k
+            // formula's value (the reference's last store) from both of its
+            // stores.  Lanes past the batch store to the sink.  A loop of
+            // run-time length or a store under a divergent branch here made
+            // the next group's wait for its prefetched input drain all of
+            // these stores first.
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
             __syncthreads();
+            const long long fb = active ? f * (M + 1) : 0;
+            void* const o0 = active ? a.out0 : a.sink;
+            void* const o1 = active ? a.out1 : const_cast<void*>(sk1);
             // pw: squared magnitude -> output value (|X|^2, or its dB level in
-            // the framed kernels; one branch per group, not per bin)
-            auto epilogue = [&](auto pw) {
-                const long long fb = f * (M + 1);
-                if (t == 0) {
-                    const float2 z0 = slds[0];
-                    const float dc = z0.x + z0.y, ny = z0.x - z0.y;
-                    if constexpr (KIND == R2C) {
-                        rst_c(a.out0, a.out1, LOUT, fb, make_float2(dc, 0.f));
-                        rst_c(a.out0, a.out1, LOUT, fb + M, make_float2(ny, 0.f));
-                    } else {
-                        float* o = reinterpret_cast<float*>(a.out0);
-                        rst(o + fb, pw(dc * dc));
-                        rst(o + fb + M, pw(ny * ny));
+            // the framed kernels); cached_tag: cached or non-temporal stores
+            auto epilogue = [&](auto pw, auto cached_tag) {
+                constexpr bool cached = decltype(cached_tag)::value;
+                auto st2 = [&](float2* q, float2 x) {
+                    if constexpr (cached) *q = x; else st_nt(q, x);
+                };
+                auto st1 = [&](float* q, float x) {
+                    if constexpr (cached) *q = x; else st_nt(q, x);
+                };
+                const float2 z0 = slds[0];
+                const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+                // bins k and M - k: put(k, X[k] or, for k = M / 2, X[M - k], X[M - k])
+                auto bins = [&](auto put) {
+#pragma unroll
+                    for (int i = 0; i < G::R / 2; ++i) {
+                        // t through an empty asm per bin pair: the compiler
+                        // cannot hoist a pair's reads above the previous pair,
+                        // so the pairs' values are not all live at once
+                        int tt = t;
+                        asm volatile("" : "+v"(tt));
+                        const int k = 1 + tt + G::T * i;
+                        const float2 z1 = slds[G::pad(k)];
+                        const float2 z2 = slds[G::pad(M - k)];
+                        const float2 w = lrtw[k - 1];
+                        const float si = z1.y + z2.y, dr = z1.x - z2.x;
+                        const float2 xm = make_float2(__builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x)),
+                                                      __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y)));
+                        const float2 xk = make_float2(__builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x)),
+                                                      __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y)));
+                        put(k, k != M - k ? xk : xm, xm);
                     }
-                }
-                for (int k = 1 + t; k <= M / 2; k += G::T) {
-                    const float2 z1 = slds[G::pad(k)];
-                    const float2 z2 = slds[G::pad(M - k)];
-                    const float2 w = lrtw[k - 1];
-                    const float si = z1.y + z2.y, dr = z1.x - z2.x;
-                    const float xr_m = __builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x));
-                    const float xi_m = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y));
-                    const float xr_k = __builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x));
-                    const float xi_k = __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y));
-                    if constexpr (KIND == R2C) {
-                        if (k != M - k) rst_c(a.out0, a.out1, LOUT, fb + k, make_float2(xr_k, xi_k));
-                        rst_c(a.out0, a.out1, LOUT, fb + (M - k), make_float2(xr_m, xi_m));
-                    } else {
-                        float* o = reinterpret_cast<float*>(a.out0);
-                        if (k != M - k) rst(o + fb + k, pw(__builtin_fmaf(xr_k, xr_k, xi_k * xi_k)));
-                        rst(o + fb + (M - k), pw(__builtin_fmaf(xr_m, xr_m, xi_m * xi_m)));
-                    }
+                };
+                if constexpr (KIND == R2C && LOUT == AOS) {
+                    float2* const o = static_cast<float2*>(o0) + fb;
+                    st2(o, make_float2(dc, 0.f));
+                    st2(o + M, make_float2(ny, 0.f));
+                    bins([&](int k, float2 xk, float2 xm) {
+                        st2(o + k, xk);
+                        st2(o + (M - k), xm);
+                    });
+                } else if constexpr (KIND == R2C) {
+                    float* const q0 = static_cast<float*>(o0) + fb;
+                    float* const q1 = static_cast<float*>(o1) + fb;
+                    st1(q0, dc);
+                    st1(q1, 0.f);
+                    st1(q0 + M, ny);
+                    st1(q1 + M, 0.f);
+                    bins([&](int k, float2 xk, float2 xm) {
+                        st1(q0 + k, xk.x);
+                        st1(q1 + k, xk.y);
+                        st1(q0 + (M - k), xm.x);
+                        st1(q1 + (M - k), xm.y);
+                    });
+                } else {
+                    float* const q = static_cast<float*>(o0) + fb;
+                    st1(q, pw(dc * dc));
+                    st1(q + M, pw(ny * ny));
+                    bins([&](int k, float2 xk, float2 xm) {
+                        st1(q + k, pw(__builtin_fmaf(xk.x, xk.x, xk.y * xk.y)));
+                        st1(q + (M - k), pw(__builtin_fmaf(xm.x, xm.x, xm.y * xm.y)));
+                    });
                 }
             };
-            if (active) {
-                if (FRAMED && KIND == R2C_SQR && a.db)
-                    epilogue([&](float q) { return power_out(q, 1, a.db_floor); });
-                else
-                    epilogue([](float q) { return q; });
-            }
+            auto with_pw = [&](auto pw) {
+                if (a.cached_out) epilogue(pw, std::true_type{}); else epilogue(pw, std::false_type{});
+            };
+            if (FRAMED && KIND == R2C_SQR && a.db)
+                with_pw([&](float q) { return power_out(q, 1, a.db_floor); });
+            else
+                with_pw([](float q) { return q; });
         }
     };
     if (g < ngroups) {
