@@ -64,7 +64,6 @@ constexpr int kMtTotal = kMtLoPadded + kMtHi;
 // (in0 / out0 at offset 0, in1 / out1 at kRaggedSinkHalf)
 constexpr long long kRaggedSinkHalf = 8ll * ((1 << kRaggedMaxOrder) + 64);
 constexpr long long kRaggedSinkBytes = 2 * kRaggedSinkHalf;
-constexpr int kRaggedWsink = 16;  // floats (one 64-byte line) per workgroup
 
 struct RArgs {
     const RDesc* desc;    // descriptors; entry `sink` points at the sink
@@ -73,7 +72,6 @@ struct RArgs {
                           // per thread, then up to runs[4b+2] at 16, then up to
                           // runs[4b+3] at 8
     const float2* mt;     // twiddle table (kMtTotal entries)
-    float* wsink;         // stores that no lane needs: kRaggedWsink floats per workgroup
     int sink;             // index of the sink descriptor
 };
 
@@ -268,7 +266,7 @@ __device__ __forceinline__ void rg_prefetch(float2 (&p)[1 << LOGR], float2& px, 
 // sink's data and store to the sink, so every lane issues the same stores.
 template <int LOGR>
 __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, const RItem& it, const RDesc& d,
-                                             float2* lds, const float2* mt, float* wsink, int tid) {
+                                             float2* lds, const float2* mt, int tid) {
     constexpr int R = 1 << LOGR;
     const int n = it.corder;
     const int M = 1 << n;
@@ -327,8 +325,8 @@ __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, 
 
     if (kind == R2C || kind == R2C_SQR) {
         // post-twiddle (reference src/common/zldsp_fft_common_execute.hpp:317-385,
-        // 480-552); every store is issued by every lane (the sink where a lane
-        // has no bin to write)
+        // 480-552); every lane issues the same stores (lanes past the item's
+        // count through the sink descriptor)
         __syncthreads();
         {
             const int tw = rg_opaque(t);
@@ -339,7 +337,9 @@ __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, 
         const float2 z0 = slds[0];
         const float dc = z0.x + z0.y, ny = z0.x - z0.y;
         const int rsh = kMtBits - 1 - n;
-        // bins k and M - k for k = 1 + t + T i; put(k, k != M - k, X[k], X[M - k])
+        // bins k and M - k for k = 1 + t + T i: put(k, X[k] or, for
+        // k = M / 2, X[M - k], X[M - k]); bin M/2 thus gets the M - k
+        // formula's value (the reference's last store) from both stores
         auto bins = [&](auto put) {
 #pragma unroll
             for (int i = 0; i < R / 2; ++i) {
@@ -353,41 +353,38 @@ __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, 
                                               __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y)));
                 const float2 xk = make_float2(__builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x)),
                                               __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y)));
-                put(k, k != M - k, xk, xm);
+                put(k, k != M - k ? xk : xm, xm);
             }
         };
-        // DC and Nyquist are thread 0's, and bin M/2 is written once, from the
-        // M - k formula (the reference's last store): the other lanes store
-        // those values to this workgroup's sink words
-        ZL_GLOBAL float* const ws = rg_gf(wsink);
+        // every lane of a transform computes DC and Nyquist from Z[0] and
+        // stores them alike, so no store sits under a branch
         if (kind == R2C && lay == AOS) {
             ZL_GLOBAL zl_f2v* const o = rg_gf2(d.out0);
-            ZL_GLOBAL zl_f2v* const s2 = rg_gf2(wsink);
-            rg_put(t == 0 ? o : s2, make_float2(dc, 0.f));
-            rg_put(t == 0 ? o + M : s2 + 1, make_float2(ny, 0.f));
-            bins([&](int k, bool two, float2 xk, float2 xm) {
-                rg_put(two ? o + k : s2 + 2, xk);
+            rg_put(o, make_float2(dc, 0.f));
+            rg_put(o + M, make_float2(ny, 0.f));
+            bins([&](int k, float2 xk, float2 xm) {
+                rg_put(o + k, xk);
                 rg_put(o + (M - k), xm);
             });
         } else if (kind == R2C) {
             ZL_GLOBAL float* const o0 = rg_gf(d.out0);
             ZL_GLOBAL float* const o1 = rg_gf(d.out1);
-            (t == 0 ? o0[0] : ws[0]) = dc;
-            (t == 0 ? o1[0] : ws[1]) = 0.f;
-            (t == 0 ? o0[M] : ws[2]) = ny;
-            (t == 0 ? o1[M] : ws[3]) = 0.f;
-            bins([&](int k, bool two, float2 xk, float2 xm) {
-                (two ? o0[k] : ws[4]) = xk.x;
-                (two ? o1[k] : ws[5]) = xk.y;
+            o0[0] = dc;
+            o1[0] = 0.f;
+            o0[M] = ny;
+            o1[M] = 0.f;
+            bins([&](int k, float2 xk, float2 xm) {
+                o0[k] = xk.x;
+                o1[k] = xk.y;
                 o0[M - k] = xm.x;
                 o1[M - k] = xm.y;
             });
         } else {
             ZL_GLOBAL float* const o = rg_gf(d.out0);
-            (t == 0 ? o[0] : ws[0]) = dc * dc;
-            (t == 0 ? o[M] : ws[1]) = ny * ny;
-            bins([&](int k, bool two, float2 xk, float2 xm) {
-                (two ? o[k] : ws[2]) = __builtin_fmaf(xk.x, xk.x, xk.y * xk.y);
+            o[0] = dc * dc;
+            o[M] = ny * ny;
+            bins([&](int k, float2 xk, float2 xm) {
+                o[k] = __builtin_fmaf(xk.x, xk.x, xk.y * xk.y);
                 o[M - k] = __builtin_fmaf(xm.x, xm.x, xm.y * xm.y);
             });
         }

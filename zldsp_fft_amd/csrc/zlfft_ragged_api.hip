@@ -228,11 +228,11 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
         hipMalloc(&r->d_items, sizeof(RItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
         hipMalloc(&r->d_runs, sizeof(int) * runs.size()) != hipSuccess ||
         hipMalloc(&r->d_mt, sizeof(float2) * kMtTotal) != hipSuccess ||
-        hipMalloc(&r->d_sink, kRaggedSinkBytes + 4ll * kRaggedWsink * grid) != hipSuccess)
+        hipMalloc(&r->d_sink, kRaggedSinkBytes) != hipSuccess)
         return cleanup(zl_fail(ZLFFT_ERR_ALLOC, "ragged hipMalloc failed"));
     // the sink descriptor: lanes without a transform read and write there
     hd[count] = RDesc{r->d_sink, r->d_sink + kRaggedSinkHalf, r->d_sink, r->d_sink + kRaggedSinkHalf};
-    e = hipMemsetAsync(r->d_sink, 0, kRaggedSinkBytes + 4ll * kRaggedWsink * grid, r->stream);
+    e = hipMemsetAsync(r->d_sink, 0, kRaggedSinkBytes, r->stream);
     if (e == hipSuccess)
         e = hipMemcpyAsync(r->d_desc, hd.data(), sizeof(RDesc) * hd.size(), hipMemcpyHostToDevice, r->stream);
     if (e == hipSuccess && !items.empty())
@@ -249,7 +249,6 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
     a.items = r->d_items;
     a.runs = r->d_runs;
     a.mt = r->d_mt;
-    a.wsink = reinterpret_cast<float*>(r->d_sink + kRaggedSinkBytes);
     a.sink = (int)count;
 
     // capture: the persistent kernel (each workgroup runs its own items, so

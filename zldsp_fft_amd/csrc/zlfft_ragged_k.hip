@@ -19,7 +19,6 @@ __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2*
     constexpr int R = 1 << LOGR;
     constexpr bool prefetch = LOGR <= 4;
     const int tid = threadIdx.x;
-    float* const ws = a.wsink + kRaggedWsink * blockIdx.x;
     // items past the run: no transform (every lane on the sink)
     auto item = [&](int i) {
         const RItem r = a.items[i < end ? i : end - 1];
@@ -51,7 +50,7 @@ __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2*
             const RItem i2 = item(i + 2);
             const RDesc d2 = rg_desc<LOGR>(a, i2, tid);
             rg_prefetch<LOGR>(p, px, in, dn, tid);
-            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, ws, tid);
+            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, tid);
             ic = in;
             dc = dn;
             in = i2;
@@ -64,7 +63,7 @@ __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2*
         for (int i = beg; i < end; ++i) {
             float2 v[R], vx = make_float2(0.f, 0.f);
             rg_prefetch<LOGR>(v, vx, ic, dc, tid);
-            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, ws, tid);
+            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, tid);
             ic = in;
             dc = dn;
             in = item(i + 2);
