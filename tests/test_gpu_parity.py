@@ -295,6 +295,14 @@ def test_streaming_batches(order):
     assert rel(Y[idx].cpu().numpy(), ref).max() <= TOL
     s64 = np.abs(ref) ** 2
     assert (np.linalg.norm(S[idx].cpu().numpy() - s64, axis=1) / np.linalg.norm(s64, axis=1)).max() <= 2 * TOL
+    # DC and Nyquist bins are real exactly, as the reference stores them; the
+    # SoA output holds the AoS output's values bit for bit (same arithmetic,
+    # other stores), the part-filled last group included
+    assert torch.count_nonzero(Y.imag[:, 0]) == 0 and torch.count_nonzero(Y.imag[:, n]) == 0
+    Yso = make_out((batch, n + 1), zf.SOA)
+    r.forward(xr, Yso)
+    torch.cuda.synchronize()
+    assert torch.equal(Yso[0], Y.real.contiguous()) and torch.equal(Yso[1], Y.imag.contiguous()), order
     # the streaming kernel's epilogue gives the bits of the one-launch LDS
     # kernel, which small batches take: first frames and the part-filled last
     # group.
