@@ -157,7 +157,9 @@ __global__ void gen_pass_twiddles(CX* out, int lr, int lw, int count) {
 // LDS twiddle image of one Stockham pass (radix 2^lr, width 2^lw): the
 // direct table for w <= 16, else lo[(m-1)16 + b] = W_{rw}^{m b} followed by
 // hi[(m-1)(w/16) + a] = W_{rw}^{16 m a}; FP64 on the GPU, rounded to FP32.
-__global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi_n) {
+template <class CX>
+__global__ void gen_image_twiddles(CX* out, int lr, int lw, int lo_n, int hi_n) {
+    using S = typename zl::CxT<CX>::S;
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= lo_n + hi_n) return;
     const long long w = 1ll << lw, rw = 1ll << (lr + lw);
@@ -172,7 +174,7 @@ __global__ void gen_image_twiddles(float2* out, int lr, int lw, int lo_n, int hi
     e &= rw - 1;
     double sn, cs;
     sincospi(-2.0 * (double)e / (double)rw, &sn, &cs);
-    out[idx] = make_float2((float)cs, (float)sn);
+    out[idx] = zl::CxT<CX>::mk((S)cs, (S)sn);
 }
 
 // MFMA kernels' twiddle table (layout at kMfmaTwTotal): W_rw^{m k} for rows
@@ -455,12 +457,12 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
     if (groups > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
     void* params[] = {&a};
     const int resident = (e.stream_fn[0] && !no_stream_kernels()) ? p->stream_blocks[kind][lin][lout] * p->cus : 0;
-    if (resident > 0 && groups > (size_t)resident) {
+    if (resident > 0 && (batch + e.s_tpw - 1) / e.s_tpw > (size_t)resident) {
         // persistent streaming kernel: one workgroup per resident slot
         a.tw = p->d_img;
         a.sink = p->d_sink;
         a.cached_out = rfft_cached_out(batch * (((size_t)1 << p->corder) + 1) * 8);
-        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident), dim3(e.wg),
+        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident), dim3(e.s_wg),
                                params, 0, reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;
     }
@@ -795,7 +797,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                     if (!ke.stream_fn[0]) continue;
                     int nb = 0;
                     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &nb, reinterpret_cast<const void*>(ke.stream_fn[0]), ke.wg, 0);
+                        &nb, reinterpret_cast<const void*>(ke.stream_fn[0]), ke.s_wg, 0);
                     if (e != hipSuccess) return cleanup(hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
                     p->stream_blocks[k][i][o] = nb;
                 }
@@ -812,13 +814,18 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         // real kinds' (one geometry and image for all of them) for RFFT plans
         const zl::KernelEntry& ki = (*p->tab)[own_kind][0][0];
         if (ki.img_total > 0) {
-            e = hipMalloc(&p->d_img, sizeof(float2) * (size_t)ki.img_total);
+            e = hipMalloc(&p->d_img, csz * (size_t)ki.img_total);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
             size_t off = 0;
-            for (int q = 1; q < ki.np; ++q) {
+            for (int q = 1; q < ki.s_np; ++q) {
                 const int cnt = ki.img_lo[q] + ki.img_hi[q];
-                gen_image_twiddles<<<(cnt + 255) / 256, 256, 0, p->stream>>>(p->d_img + off, ki.lr[q], ki.lw[q],
-                                                                              ki.img_lo[q], ki.img_hi[q]);
+                const unsigned g = (unsigned)((cnt + 255) / 256);
+                if (f64)
+                    gen_image_twiddles<<<g, 256, 0, p->stream>>>(reinterpret_cast<double2*>(p->d_img) + off,
+                                                                  ki.s_lr[q], ki.s_lw[q], ki.img_lo[q], ki.img_hi[q]);
+                else
+                    gen_image_twiddles<<<g, 256, 0, p->stream>>>(p->d_img + off, ki.s_lr[q], ki.s_lw[q], ki.img_lo[q],
+                                                                  ki.img_hi[q]);
                 off += (size_t)cnt;
             }
         }
@@ -1046,13 +1053,13 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
         void* params[] = {&a};
         const int fk = kind == zl::R2C ? zl::R2C_FRAMED : zl::R2C_SQR_FRAMED;
         const int resident = (e.stream_fn[0] && !no_stream_kernels()) ? p->stream_blocks[fk][zl::AOS][lout] * p->cus : 0;
-        if (resident > 0 && groups > (size_t)resident) {
+        if (resident > 0 && (nframes + e.s_tpw - 1) / e.s_tpw > (size_t)resident) {
             // persistent streaming kernel, as in launch()
             a.tw = p->d_img;
             a.sink = p->d_sink;
             a.cached_out = rfft_cached_out(nframes * (((size_t)1 << p->corder) + 1) * 8);
             ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[0]), dim3((unsigned)resident),
-                                   dim3(e.wg), params, 0, reinterpret_cast<hipStream_t>(stream)));
+                                   dim3(e.s_wg), params, 0, reinterpret_cast<hipStream_t>(stream)));
             return ZLFFT_OK;
         }
         ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params, 0,

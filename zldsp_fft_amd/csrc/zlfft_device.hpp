@@ -663,6 +663,11 @@ __global__ void __launch_bounds__(G::WG) fft_lds_kernel(KArgs a) {
 // for the RFFT and STFT kernels, whose LDS also holds the RFFT twiddles and
 // the window, and for orders 13-14, where the table outgrows the LDS budget.
 constexpr int cfg_tw_direct_max(int logn, int kind) { return kind == C2C && logn >= 9 && logn <= 12 ? 4096 : 16; }
+// The same for the streaming kernel over complex type CX: FP64 keeps every
+// pass above width 16 factored, so that two workgroups' tiles and images fit
+// the LDS of one CU.
+template <class CX>
+constexpr int cfg_stream_direct_max(int logn, int kind) { return sizeof(CX) == 8 ? cfg_tw_direct_max(logn, kind) : 16; }
 
 // LDS image of the Stockham twiddles of geometry G.  Pass p (radix r, width w)
 // needs W_{r w}^{m k}, m = 1..r-1, k < w.  For w <= DMAX the image holds them
@@ -686,26 +691,26 @@ struct TwImage {
 };
 
 // Pass P with twiddles from the LDS image `ltw` (see TwImage).
-template <class G, int P, int DMAX>
-__device__ __forceinline__ void run_pass_img(float2 (&v)[G::R], float2* slds, const float2* ltw, int t) {
+template <class G, int P, int DMAX, class CX = float2>
+__device__ __forceinline__ void run_pass_img(CX (&v)[G::R], CX* slds, const CX* ltw, int t) {
     constexpr int lr = G::lr(P), r = 1 << lr, nb = G::R / r, lw = G::lw(P), w = 1 << lw;
     constexpr bool last = (P == G::NP - 1);
     using TI = TwImage<G, DMAX>;
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
-        float2 x[r];
+        CX x[r];
 #pragma unroll
         for (int m = 0; m < r; ++m) x[m] = v[b + m * nb];
         if constexpr (P > 0) {
             const int i = t + G::T * b;
             const int k = i & (w - 1);
             if constexpr (w <= DMAX) {
-                const float2* twp = ltw + TI::off(P) + k;
+                const CX* twp = ltw + TI::off(P) + k;
 #pragma unroll
                 for (int m = 1; m < r; ++m) x[m] = cmul(x[m], twp[(m - 1) * w]);
             } else {
-                const float2* lo = ltw + TI::off(P) + (k & 15);
-                const float2* hi = ltw + TI::off(P) + TI::lo_n(P) + (k >> 4);
+                const CX* lo = ltw + TI::off(P) + (k & 15);
+                const CX* hi = ltw + TI::off(P) + TI::lo_n(P) + (k >> 4);
 #pragma unroll
                 for (int m = 1; m < r; ++m) x[m] = cmul(x[m], cmul(lo[(m - 1) * 16], hi[(m - 1) * (w / 16)]));
             }
@@ -726,7 +731,7 @@ __device__ __forceinline__ void run_pass_img(float2 (&v)[G::R], float2* slds, co
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
-        run_pass_img<G, P + 1, DMAX>(v, slds, ltw, t);
+        run_pass_img<G, P + 1, DMAX, CX>(v, slds, ltw, t);
     }
 }
 
@@ -745,20 +750,21 @@ __device__ __forceinline__ void run_pass_img(float2 (&v)[G::R], float2* slds, co
 #ifndef ZL_STREAM_MIN_WAVES
 #define ZL_STREAM_MIN_WAVES 1
 #endif
-template <class G, int KIND, int LIN, int LOUT, int BWD, int FRAMED = 0>
+template <class G, int KIND, int LIN, int LOUT, int BWD, int FRAMED = 0, class CX = float2>
 __global__ void __launch_bounds__(G::WG) __attribute__((amdgpu_waves_per_eu(ZL_STREAM_MIN_WAVES, 8)))
 fft_stream_kernel(KArgs a) {
     static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR, "streaming kernel handles C2C and R2C");
     static_assert(!FRAMED || KIND != C2C, "framed input is real");
-    constexpr int DMAX = cfg_tw_direct_max(G::logn, KIND);
+    static_assert(sizeof(CX) == 8 || KIND == C2C, "FP64 streaming kernel: C2C only");
+    constexpr int DMAX = cfg_stream_direct_max<CX>(G::logn, KIND);
     using TI = TwImage<G, DMAX>;
-    __shared__ float2 lds[G::TPW * G::LDSN];
-    __shared__ float2 ltw[TI::total() > 0 ? TI::total() : 1];
-    __shared__ float2 lrtw[KIND == C2C ? 1 : (G::N / 2 > 0 ? G::N / 2 : 1)];
+    __shared__ CX lds[G::TPW * G::LDSN];
+    __shared__ CX ltw[TI::total() > 0 ? TI::total() : 1];
+    __shared__ CX lrtw[KIND == C2C ? 1 : (G::N / 2 > 0 ? G::N / 2 : 1)];
     const int tid = threadIdx.x;
     const int slot = tid / G::T;
     const int t = tid - slot * G::T;
-    float2* slds = lds + slot * G::LDSN;
+    CX* slds = lds + slot * G::LDSN;
     constexpr int M = G::N;
     constexpr int lin = (KIND == C2C) ? LIN : AOS;
     constexpr bool bwd = (KIND == C2C) && BWD != 0;  // direction fixed per instantiation
@@ -771,16 +777,17 @@ fft_stream_kernel(KArgs a) {
     constexpr int NI = (TI::total() + G::WG - 1) / G::WG;
     constexpr int NR = KIND == C2C ? 0 : (M / 2 + G::WG - 1) / G::WG;
     constexpr int NW = FRAMED ? (M + G::WG - 1) / G::WG : 0;
-    float2 im[NI > 0 ? NI : 1], rr[NR > 0 ? NR : 1], ww[NW > 0 ? NW : 1];
+    CX im[NI > 0 ? NI : 1], rr[NR > 0 ? NR : 1];
+    float2 ww[NW > 0 ? NW : 1];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
         const int e = tid + G::WG * i;
-        if (e < TI::total()) im[i] = static_cast<const float2*>(a.tw)[e];
+        if (e < TI::total()) im[i] = static_cast<const CX*>(a.tw)[e];
     }
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         const int e = tid + G::WG * i;
-        if (e < M / 2) rr[i] = static_cast<const float2*>(a.rtw)[e];
+        if (e < M / 2) rr[i] = static_cast<const CX*>(a.rtw)[e];
     }
     if constexpr (FRAMED) {
         // the window as M pairs (1 without a window: x * 1 = x)
@@ -800,7 +807,7 @@ fft_stream_kernel(KArgs a) {
     const void* sk1 = reinterpret_cast<const float*>(a.sink) + (1 << 14);
     // prefetch of frame / transform f into p (the sink where !ok); a framed
     // lane reads floats [0, 2 M) of the sink, M <= 2^14 (the sink holds 2^16)
-    auto prefetch = [&](float2 (&p)[G::R], long long f, bool ok) {
+    auto prefetch = [&](CX (&p)[G::R], long long f, bool ok) {
         if constexpr (FRAMED) {
             const float* x = static_cast<const float*>(ok ? a.in0 : sk0) + (ok ? f * a.hop : 0);
 #pragma unroll
@@ -813,13 +820,13 @@ fft_stream_kernel(KArgs a) {
             const void* b1 = ok ? a.in1 : sk1;
             const long long eo = ok ? f * M : 0;
 #pragma unroll
-            for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt(b0, b1, lin, eo + t + G::T * j);
+            for (int j = 0; j < G::R; ++j) p[j] = ld_c_nt<CX>(b0, b1, lin, eo + t + G::T * j);
         }
     };
     // the window in LDS (kept in registers instead it cost 32 VGPRs and a
     // third of the occupancy)
     __shared__ float2 lwin[FRAMED ? M : 1];
-    float2 p[G::R];
+    CX p[G::R];
     long long g = blockIdx.x;
     {
         const long long f = g * G::TPW + slot;
@@ -848,7 +855,7 @@ fft_stream_kernel(KArgs a) {
     auto group = [&](long long g) {
         const long long f = g * G::TPW + slot;
         const bool active = f < a.batch;
-        float2 v[G::R];
+        CX v[G::R];
 #pragma unroll
         for (int j = 0; j < G::R; ++j) v[j] = bwd ? swapri(p[j]) : p[j];
         {

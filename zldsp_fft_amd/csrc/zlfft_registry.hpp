@@ -23,7 +23,11 @@ struct KernelEntry {
     int lr[8];     // log2 radix per pass
     int lw[8];     // log2 width per pass
     int tw_total;  // complex twiddles of all passes (direct table)
-    int img_total; // complex twiddles of the LDS image (streaming variant)
+    // the streaming variant's geometry (the LDS kernel's, except FP64 C2C,
+    // whose streaming variant runs 8 points per thread) and its LDS image
+    int s_wg, s_tpw, s_np;
+    int s_lr[8], s_lw[8];
+    int img_total; // complex twiddles of the LDS image
     int img_lo[8]; // per pass: entries of the direct / low factor part
     int img_hi[8]; // per pass: entries of the high factor part (0 if direct)
 };
@@ -32,18 +36,28 @@ struct KernelEntry {
 // R2C_FRAMED / R2C_SQR_FRAMED); unused combinations stay null
 using OrderTable = KernelEntry[6][2][2];
 
-// The persistent streaming variant (FP32), for geometries whose direct I/O is
-// coalesced and whose LDS (transforms, twiddle image, RFFT twiddles) fits.
-template <class G, int KIND, int LIN, int LOUT, int FRAMED = 0>
+// The persistent streaming variant, for geometries whose direct I/O is
+// coalesced and whose LDS (transforms, twiddle image, RFFT twiddles) fits:
+// FP32, and FP64 C2C where two workgroups fit one CU.
+template <class G, int KIND, int LIN, int LOUT, int FRAMED = 0, class CX = float2>
 void add_stream_variant(KernelEntry& e) {
-    constexpr int elem = (KIND == C2C && (LIN == SOA || LOUT == SOA)) ? 4 : 8;
+    constexpr int cs = (int)sizeof(CX);
+    constexpr int elem = (KIND == C2C && (LIN == SOA || LOUT == SOA)) ? cs / 2 : cs;
     constexpr bool direct = G::T * elem >= 128 || G::TPW == 1;
-    using TI = TwImage<G, cfg_tw_direct_max(G::logn, KIND)>;
-    constexpr long long stream_lds = 8ll * (G::TPW * G::LDSN + TI::total() + (KIND == C2C ? 1 : G::N / 2) +
-                                            (FRAMED ? G::N : 0));
-    if constexpr (KIND != C2R && direct && G::NP >= 2 && stream_lds <= 160 * 1024) {
-        e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED>;
-        if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1>;
+    using TI = TwImage<G, cfg_stream_direct_max<CX>(G::logn, KIND)>;
+    constexpr long long stream_lds = (long long)cs * (G::TPW * G::LDSN + TI::total() + (KIND == C2C ? 1 : G::N / 2) +
+                                                      (FRAMED ? G::N : 0));
+    constexpr bool fits = cs == 8 ? stream_lds <= 160 * 1024 : KIND == C2C && !FRAMED && 2 * stream_lds <= 160 * 1024;
+    if constexpr (KIND != C2R && direct && G::NP >= 2 && fits) {
+        e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED, CX>;
+        if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1, 0, CX>;
+        e.s_wg = G::WG;
+        e.s_tpw = G::TPW;
+        e.s_np = G::NP;
+        for (int p = 0; p < G::NP && p < 8; ++p) {
+            e.s_lr[p] = G::lr(p);
+            e.s_lw[p] = G::lw(p);
+        }
         e.img_total = TI::total();
         for (int p = 1; p < G::NP && p < 8; ++p) {
             e.img_lo[p] = TI::lo_n(p);
@@ -69,9 +83,10 @@ KernelEntry make_entry() {
     return e;
 }
 
-// Entry over complex type CX: FP64 (LDS kernel only), and the framed STFT
-// kernels of both precisions (FP32 with the streaming variant)
-template <class G, int KIND, int LIN, int LOUT, class CX = double2, int FRAMED = 0>
+// Entry over complex type CX: FP64 (the LDS kernel, and for C2C the
+// streaming variant of geometry GS), and the framed STFT kernels of both
+// precisions (FP32 with the streaming variant)
+template <class G, int KIND, int LIN, int LOUT, class CX = double2, int FRAMED = 0, class GS = G>
 KernelEntry make_lds_entry() {
     static_assert((long long)sizeof(CX) * G::TPW * G::LDSN <= 160 * 1024, "geometry exceeds the LDS");
     KernelEntry e{};
@@ -86,6 +101,7 @@ KernelEntry make_lds_entry() {
     }
     e.tw_total = G::tw_total();
     if constexpr (FRAMED && sizeof(CX) == 8) add_stream_variant<G, KIND, LIN, LOUT, FRAMED>(e);
+    if constexpr (!FRAMED && sizeof(CX) == 16 && KIND == C2C) add_stream_variant<GS, KIND, LIN, LOUT, 0, CX>(e);
     return e;
 }
 
