@@ -59,6 +59,33 @@ def test_cfft64_lds_orders(order):
     print(f"f64 cfft order {order}: max relL2 vs FP64 oracle {worst:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("order", [5, 8, 10, 12, 13])
+def test_cfft64_streaming_batches(order):
+    """Batches larger than the resident workgroups (the persistent FP64
+    streaming kernel, 8 points per thread, where two workgroups fit a CU),
+    every layout pair, both directions, a part-filled last group: sampled
+    transforms against the FP64 oracle."""
+    n = 1 << order
+    batch = max(2048, (1 << 21) // n) + 3
+    g = torch.Generator(device="cuda").manual_seed(200 + order)
+    x = torch.randn((batch, n), dtype=torch.complex128, device="cuda", generator=g)
+    plan = zf.CFFT(order, zf.PREC_F64)
+    idx = np.unique(np.linspace(0, batch - 1, 10).astype(int))
+    xs = x[idx].cpu().numpy()
+    worst = 0.0
+    for forward in (True, False):
+        ref = np.stack([oracle.cfft(r, forward, "f64") for r in xs])
+        for lin, lout in LAYOUTS:
+            xin = x if lin == zf.AOS else (x.real.contiguous(), x.imag.contiguous())
+            y = make_out((batch, n), lout)
+            (plan.forward if forward else plan.backward)(xin, y)
+            got = fetch(tuple(t[idx] for t in y) if isinstance(y, tuple) else y[idx])
+            e = signals.rel_l2(got, ref).max()
+            worst = max(worst, e)
+            assert e <= TOL, (order, forward, lin, lout, e)
+    print(f"f64 streaming order {order} batch {batch}: max relL2 vs FP64 oracle {worst:.2e}", flush=True)
+
+
 def test_cfft64_golden_fixtures(fixtures):
     """Against the long-double brute-force DFT fixtures (orders 0-10)."""
     for order in range(0, 11):
