@@ -8,7 +8,13 @@ to zlfft_tune.txt next to libzlfft.so, where plans read it.
 For every order, a batch of one size that cycles through every variant
 (CFFT / RFFT x forward / backward x AoS / SoA, sqr-mag) is timed with each
 legal choice; the choices' outputs must agree (rel-L2 <= 1e-6) before one is
-kept.
+kept.  Then the table is refined on a mixed batch (10,000 transforms, orders
+uniform in 6..14, every variant): largest order first, each other legal
+choice is timed in the mix and kept when it is more than 1% faster.  In one
+launch, the orders share the workgroups' runs (items at 32 points per thread
+run without the prefetch, the others with it), so an order's best choice
+alone is not always its best choice in a mix: at order 13, 16 points per
+thread were faster alone and 32 points faster in the mix by 6-9%.
 """
 from __future__ import annotations
 
@@ -45,7 +51,7 @@ def _rel(a, b):
     return float((num / den).max())
 
 
-def tune(points: int, out_path: str, verbose: bool = True) -> dict:
+def tune(points: int, out_path: str, verbose: bool = True, mix_count: int = 10000) -> dict:
     import torch
     results = {}
     base = [default_logr(n) for n in range(15)]
@@ -76,24 +82,69 @@ def tune(points: int, out_path: str, verbose: bool = True) -> dict:
         if verbose:
             print(f"order {order:2d}: " + ", ".join(f"{1 << k:2d} pts/thread {v:.4f} ms" for k, v in sorted(times.items()))
                   + f" -> {1 << best}", flush=True)
+    table = [default_logr(n) for n in range(15)]
+    for order, (best, _) in results.items():
+        table[order] = best
+    mixed = refine_on_mix(table, verbose, count=mix_count) if mix_count else {}
     props = torch.cuda.get_device_properties(0)
     with open(out_path, "w") as f:
         f.write("# zlfft tuning table, written by python -m zldsp_fft_amd.autotune\n")
         f.write(f"# device: {props.name} ({getattr(props, 'gcnArchName', '?')}), {props.multi_processor_count} CUs\n")
         f.write(f"# batch per order: {points} points cycling through every transform variant\n")
+        if mixed:
+            f.write(f"# refined on a mix of {mix_count} transforms, orders 6..14 (mix: ms per launch)\n")
         f.write("# key order value   (ragged_logr: log2 points per thread of the ragged kernel)\n")
         for order, (best, times) in sorted(results.items()):
             detail = " ".join(f"{1 << k}pts={v:.4f}ms" for k, v in sorted(times.items()))
-            f.write(f"ragged_logr {order} {best}   # {detail}\n")
+            if order in mixed:
+                detail += "  mix: " + " ".join(f"{1 << k}pts={v:.4f}ms" for k, v in sorted(mixed[order].items()))
+            f.write(f"ragged_logr {order} {table[order]}   # {detail}\n")
     return results
+
+
+def refine_on_mix(table: list, verbose: bool = True, count: int = 10000, seed: int = 99) -> dict:
+    """Coordinate pass over orders 14..6 on one mixed batch: each other legal
+    choice of an order is timed with the rest of `table` fixed, and kept
+    (in `table`) when more than 1% faster.  Returns {order: {logr: ms}}."""
+    import torch
+    descs = workloads.ragged_mix(count, seed)
+
+    def run_ms(t):
+        rb = zf.RaggedBatch(descs, logr=t)
+        ms = min(workloads.time_ms(torch, rb.run, reps=20, warmup=5) for _ in range(2))
+        rb.close()
+        return ms
+
+    cur = run_ms(table)
+    out = {}
+    for order in range(14, 5, -1):
+        times = {table[order]: cur}
+        for logr in LOGRS:
+            if logr == table[order] or not legal(order, logr):
+                continue
+            t = list(table)
+            t[order] = logr
+            times[logr] = run_ms(t)
+        best = min(times, key=times.get)
+        if best != table[order] and times[best] < 0.99 * cur:
+            table[order] = best
+            cur = times[best]
+        out[order] = times
+        if verbose:
+            print(f"mix, order {order:2d}: " + ", ".join(f"{1 << k:2d} pts/thread {v:.4f} ms" for k, v in sorted(times.items()))
+                  + f" -> {1 << table[order]}", flush=True)
+    del descs
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "zlfft_tune.txt"))
     ap.add_argument("--points", type=int, default=1 << 24)
+    ap.add_argument("--mix", type=int, default=10000, help="transforms in the refinement mix (0: no refinement)")
     args = ap.parse_args()
-    tune(args.points, args.out)
+    tune(args.points, args.out, mix_count=args.mix)
     print("wrote", args.out)
 
 
