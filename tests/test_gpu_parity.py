@@ -262,7 +262,7 @@ def test_batched_4096_point_config_properties():
     assert rel(ys, oracle_rows(xs, True, "f64")).max() <= TOL
 
 
-@pytest.mark.parametrize("order", [5, 8, 9, 10, 11, 12, 13, 14])
+@pytest.mark.parametrize("order", [5, 6, 7, 8, 9, 10, 11, 12, 13, 14])
 def test_streaming_batches(order):
     """Batches larger than the resident workgroups run the persistent
     streaming kernel; every transform must match the oracle."""
@@ -308,8 +308,10 @@ def test_streaming_batches(order):
     # group.
     # Up to order 8 both kernels read every pass twiddle from a direct table;
     # above it the streaming kernel's compact LDS image forms some twiddles as
-    # a product of two factors, so only the tolerance above applies there.
-    for lo, hi in ((0, 8), (batch - 11, batch)) if order <= 8 else ():
+    # a product of two factors, and at orders 6-7 it runs 4 / 8 points per
+    # thread (radix-4 / 8 passes) where the one-launch kernel runs 16, so only
+    # the tolerance above applies there.
+    for lo, hi in ((0, 8), (batch - 11, batch)) if order <= 8 and order not in (6, 7) else ():
         Ys = torch.empty((hi - lo, n + 1), dtype=torch.complex64, device="cuda")
         Ss = torch.empty((hi - lo, n + 1), dtype=torch.float32, device="cuda")
         r.forward(xr[lo:hi], Ys)

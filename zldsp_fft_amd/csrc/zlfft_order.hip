@@ -8,19 +8,29 @@
 #error "compile with -DZL_LOGN=<order>"
 #endif
 
+// Points per thread (log2) of the FP32 streaming variants: the LDS
+// kernel's, except orders 6 and 7, where 16 points per thread leave 4 or 8
+// threads per transform, too few for coalesced direct I/O: there 4 and 8
+// points (16 threads, 128-byte rows) let the AoS batches stream (measured,
+// 2^26 points: CFFT order 6 3950 -> 5540 GB/s, order 7 3890 -> 5860).
+#ifndef ZL_STREAM_LOGR
+#define ZL_STREAM_LOGR(n) ((n) == 6 ? 2 : (n) == 7 ? 3 : cfg_logr(n))
+#endif
+
 #define ZL_CAT2(a, b) a##b
 #define ZL_CAT(a, b) ZL_CAT2(a, b)
 
 void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
     using namespace zl;
     using G = Geo<ZL_LOGN, cfg_logr(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
-    tab[C2C][AOS][AOS] = make_entry<G, C2C, AOS, AOS>();
-    tab[C2C][AOS][SOA] = make_entry<G, C2C, AOS, SOA>();
-    tab[C2C][SOA][AOS] = make_entry<G, C2C, SOA, AOS>();
-    tab[C2C][SOA][SOA] = make_entry<G, C2C, SOA, SOA>();
-    tab[R2C][AOS][AOS] = make_entry<G, R2C, AOS, AOS>();
-    tab[R2C][AOS][SOA] = make_entry<G, R2C, AOS, SOA>();
-    tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS>();
+    using GS = Geo<ZL_LOGN, ZL_STREAM_LOGR(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
+    tab[C2C][AOS][AOS] = make_entry<G, C2C, AOS, AOS, GS>();
+    tab[C2C][AOS][SOA] = make_entry<G, C2C, AOS, SOA, GS>();
+    tab[C2C][SOA][AOS] = make_entry<G, C2C, SOA, AOS, GS>();
+    tab[C2C][SOA][SOA] = make_entry<G, C2C, SOA, SOA, GS>();
+    tab[R2C][AOS][AOS] = make_entry<G, R2C, AOS, AOS, GS>();
+    tab[R2C][AOS][SOA] = make_entry<G, R2C, AOS, SOA, GS>();
+    tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS, GS>();
     tab[C2R][AOS][AOS] = make_entry<G, C2R, AOS, AOS>();
     tab[C2R][SOA][AOS] = make_entry<G, C2R, SOA, AOS>();
     tab[R2C_FRAMED][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, float2, 1>();

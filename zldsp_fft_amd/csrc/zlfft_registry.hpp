@@ -66,7 +66,7 @@ void add_stream_variant(KernelEntry& e) {
     }
 }
 
-template <class G, int KIND, int LIN, int LOUT>
+template <class G, int KIND, int LIN, int LOUT, class GS = G>
 KernelEntry make_entry() {
     KernelEntry e{};
     e.fn = fft_lds_kernel<G, KIND, LIN, LOUT>;
@@ -79,7 +79,7 @@ KernelEntry make_entry() {
         e.lw[p] = G::lw(p);
     }
     e.tw_total = G::tw_total();
-    add_stream_variant<G, KIND, LIN, LOUT>(e);
+    add_stream_variant<GS, KIND, LIN, LOUT>(e);
     return e;
 }
 
