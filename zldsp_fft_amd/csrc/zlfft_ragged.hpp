@@ -264,9 +264,12 @@ __device__ __forceinline__ void rg_prefetch(float2 (&p)[1 << LOGR], float2& px, 
 // then radix R) is walked at run time.  Same arithmetic as fft_lds_kernel
 // apart from the twiddles' rounding.  Lanes past the item's count run on the
 // sink's data and store to the sink, so every lane issues the same stores.
-template <int LOGR>
+// after_passes() runs once the passes are done, before the epilogue's stores:
+// loads it issues are ahead of the stores in the memory counter, so waiting
+// for them later does not wait for this item's stores.
+template <int LOGR, class AfterPasses>
 __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, const RItem& it, const RDesc& d,
-                                             float2* lds, const float2* mt, int tid) {
+                                             float2* lds, const float2* mt, int tid, AfterPasses after_passes) {
     constexpr int R = 1 << LOGR;
     const int n = it.corder;
     const int M = 1 << n;
@@ -322,6 +325,7 @@ __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, 
     }
 
     rgc_passes<LOGR>(n, v, slds, mt, t);
+    after_passes();
 
     if (kind == R2C || kind == R2C_SQR) {
         // post-twiddle (reference src/common/zldsp_fft_common_execute.hpp:317-385,

@@ -10,9 +10,10 @@ namespace zl {
 // then transforms and stores item i.  The first step is peeled off the loop so
 // the loop header is reached only from a copy of the loop body: both ways in
 // then have the same loads and stores in flight, and the header's wait covers
-// the input loads only.  With 32 points per thread (CFFT order 14) the
-// transform alone takes most of the 256 VGPRs, so an item's input is loaded
-// at its own step, with its descriptors loaded one step ahead.
+// the input loads only.  With 32 points per thread the passes alone take most
+// of the 256 VGPRs, so item i + 1's input is loaded after item i's passes,
+// ahead of item i's stores (its descriptors one step ahead): the next step's
+// wait for it then leaves those stores in flight.
 template <int LOGR>
 __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2* lds, float2* mt,
                                        const float2 (&mt_reg)[2], bool fill_mt) {
@@ -50,7 +51,7 @@ __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2*
             const RItem i2 = item(i + 2);
             const RDesc d2 = rg_desc<LOGR>(a, i2, tid);
             rg_prefetch<LOGR>(p, px, in, dn, tid);
-            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, tid);
+            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, tid, []() {});
             ic = in;
             dc = dn;
             in = i2;
@@ -59,11 +60,20 @@ __device__ __forceinline__ void rg_run(const RArgs& a, int beg, int end, float2*
         step(beg);
         for (int i = beg + 1; i < end; ++i) step(i);
     } else {
+        float2 p[R], px = make_float2(0.f, 0.f);
+        rg_prefetch<LOGR>(p, px, ic, dc, tid);
         fill();
         for (int i = beg; i < end; ++i) {
-            float2 v[R], vx = make_float2(0.f, 0.f);
-            rg_prefetch<LOGR>(v, vx, ic, dc, tid);
-            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, tid);
+            float2 v[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[j] = p[j];
+            const float2 vx = px;
+            // the loads stay between the passes and the stores
+            rg_transform<LOGR>(v, vx, ic, dc, lds, mt, tid, [&]() {
+                __builtin_amdgcn_sched_barrier(0);
+                rg_prefetch<LOGR>(p, px, in, dn, tid);
+                __builtin_amdgcn_sched_barrier(0);
+            });
             ic = in;
             dc = dn;
             in = item(i + 2);
