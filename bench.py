@@ -3,15 +3,28 @@
 Workload (one "step"): one forward pass of 16384 complex FP32 CFFTs of
 N = 4096 points, AoS in and AoS out (1 GiB of algorithmic HBM traffic:
 16 B per point, one read of the input and one write of the output), with the
-inputs resident in HBM.  The step is one launch of the LDS-resident kernel
-through the C-ABI (zlfft_cfft_exec) on torch's current stream: the persistent
-streaming Stockham kernel for N = 4096.
+inputs resident in HBM.  The step is one launch through the C-ABI
+(zlfft_cfft_exec) on torch's current stream: the persistent streaming
+Stockham kernel for N = 4096.
 
 Printed: ONE JSON line with whole-job throughput in GB/s of algorithmic
 traffic, the kernel's roofline fraction against the 8 TB/s HBM peak (average
 launch duration from HIP events bracketing every launch in the timed region),
 the accuracy of the timed output against the FP64 oracle, and a CPU baseline:
 the oracle's FP32 restatement of the reference algorithm, timed on this host.
+
+Timing rules, for every line:
+* warm-up by time: back-to-back launches for at least WARM_S seconds of GPU
+  work (and at least --warmup launches for the headline), with no
+  synchronisation between the warm-up and the timed launches, so every line
+  starts at the same clock state;
+* the MFMA variants and the plain FP32 kernel are timed interleaved (plain,
+  variant 1, ..., variant 5, plain, ...) in short windows with an event pair
+  around every launch, at least 100 launches each; their medians are
+  reported, so the comparison does not depend on the order of measurement;
+* the secondary lines call the C-ABI directly (no Python wrapper per call),
+  rotate over buffer sets larger than the 256 MB MALL, and report both the
+  wall rate of back-to-back launches and the mean of per-launch event times.
 """
 from __future__ import annotations
 
@@ -32,6 +45,9 @@ ORDER, BATCH = 12, 16384
 N = 1 << ORDER
 BYTES_PER_STEP = 16 * N * BATCH  # read N complex64 + write N complex64, per transform
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cfft4096_b16384_pmc.md")
+WARM_S = 0.3          # seconds of back-to-back launches before a line is timed
+MIN_TIMED = 100       # launches per secondary line / MFMA variant, whatever --steps is
+ROTATE = 5            # buffer sets per secondary line: > 256 MiB, so the MALL cannot hold the working set
 
 
 def parse():
@@ -39,14 +55,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget")
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline sampling budget (T = all)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA split-precision variants")
     ap.add_argument("--no-configs", action="store_true", help="skip the analyzer and large-order lines")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch line")
-    ap.add_argument("--only", choices=("main", "mfma", "configs", "ragged"), default=None,
+    ap.add_argument("--only", choices=("main", "mfma", "configs", "ragged", "large", "analyzer"), default=None,
                     help="run one section alone (for profiler runs): the headline batch, the MFMA variants, "
-                         "the analyzer / large-order lines or the ragged batch")
+                         "the secondary lines, the ragged batch, the large-order lines or the analyzer lines")
     ap.add_argument("--mfma-precisions", default=None,
                     help="comma-separated subset of the MFMA variant names")
     ap.add_argument("--copy-probe", action="store_true",
@@ -54,79 +70,230 @@ def parse():
     return ap.parse_args()
 
 
+# ---------------------------------------------------------------- timing helpers
+
+def warm_up(torch, fns, seconds=WARM_S, min_calls=0):
+    """Back-to-back calls of fns (in turn) for about `seconds` of GPU work and
+    at least min_calls calls.  Ends without a synchronisation, so the timed
+    launches queued next follow the warm-up with no idle gap."""
+    probe = max(3, len(fns))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(probe):
+        fns[i % len(fns)]()
+    torch.cuda.synchronize()
+    per = max((time.perf_counter() - t0) / probe, 1e-6)
+    calls = max(min_calls, min(50000, int(math.ceil(seconds / per))))
+    for i in range(calls):
+        fns[i % len(fns)]()
+
+
+def time_window(torch, stream, fns, reps):
+    """ms per call over `reps` back-to-back calls (fns in turn), one event pair
+    around the window on `stream`."""
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for i in range(reps):
+        fns[i % len(fns)]()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def time_each(torch, stream, fns, reps):
+    """Per-call ms of `reps` calls (fns in turn), an event pair around each."""
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    for i in range(reps):
+        ev[i][0].record(stream)
+        fns[i % len(fns)]()
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    return [a.elapsed_time(b) for a, b in ev]
+
+
+def _line(nbytes, flops, count, ms, rel=None, kern_ms=None):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    d = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+         "transforms_per_s": round(count / (ms * 1e-3), 1), "gflops": round(flops / (ms * 1e-3) / 1e9, 1)}
+    if kern_ms is not None:
+        kg = nbytes / (kern_ms * 1e-3) / 1e9
+        d["kernel_ms_mean"] = round(kern_ms, 5)
+        d["kernel_GB/s"] = round(kg, 1)
+        d["kernel_frac_hbm"] = round(kg / HBM_PEAK_GBS, 4)
+    if rel is not None:
+        d["rel_l2_vs_fp64"] = rel
+    return d
+
+
+def measure_line(torch, stream, fns, reps=MIN_TIMED):
+    """Warm-up by time, then (wall ms per call of back-to-back launches,
+    mean of per-launch event ms)."""
+    warm_up(torch, fns)
+    wall = time_window(torch, stream, fns, reps)
+    warm_up(torch, fns, seconds=0.05)
+    each = time_each(torch, stream, fns, reps)
+    return wall, float(np.mean(each))
+
+
+def _call(lib, fn, *args):
+    rc = fn(*args)
+    if rc:
+        raise RuntimeError(lib.zlfft_last_error().decode())
+
+
+# ---------------------------------------------------------------- CPU baseline
+
+def host_cpus() -> dict:
+    """CPU model, the CPUs this process may run on, and the cgroup CPU quota."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(math.floor(quota + 1e-9))))
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity": affinity, "cgroup_quota": quota,
+            "usable": usable}
+
+
 def cpu_baseline(seconds: float) -> dict:
-    """The oracle's FP32 restatement of the reference CFFT<float> on this host:
-    a contiguous slice of the batch per thread, one workspace per thread."""
+    """The oracle's FP32 restatement of the reference CFFT<float> (a scalar C
+    port: the reference itself needs Google Highway, which is not installed)
+    on this host: a contiguous slice of the batch per thread, one workspace per
+    thread (reference instances are not shareable,
+    src/common/zldsp_fft_common_init.hpp:24-43).  T = every usable host CPU,
+    and T = 1."""
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
-    sample = 2048  # transforms per timed run (1/8 of the batch)
-    x = np.random.default_rng(0).standard_normal((sample, 2 * N)).astype(np.float32)
-    y = np.empty_like(x)
-    oracle.cfft_batch_raw(ORDER, x, y, True, "f32", threads)  # warm-up
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t0 = time.perf_counter()
-        oracle.cfft_batch_raw(ORDER, x, y, True, "f32", threads)
-        times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
+    cpus = host_cpus()
+
+    def run(threads, sample, secs):
+        x = np.random.default_rng(0).standard_normal((sample, 2 * N)).astype(np.float32)
+        y = np.empty_like(x)
+        oracle.cfft_batch_raw(ORDER, x, y, True, "f32", threads)  # warm-up
+        times = []
+        t_end = time.perf_counter() + secs
+        while time.perf_counter() < t_end or len(times) < 3:
+            t0 = time.perf_counter()
+            oracle.cfft_batch_raw(ORDER, x, y, True, "f32", threads)
+            times.append(time.perf_counter() - t0)
+        t = float(np.median(times))
+        return t, len(times)
+
+    t_all = cpus["usable"]
+    sample = max(256, 128 * t_all)
+    t, nrun = run(t_all, sample, seconds)
+    t1, nrun1 = run(1, 128, max(1.0, seconds / 2))
     l1, sw = oracle.switch_order("f32")
     path = "stockham" if ORDER < sw else "dif-macro"
     return {
         "value": round(16 * N * sample / t / 1e9, 3),
         "unit": "GB/s",
-        "cores": threads,
+        "cores": t_all,
         "kind": "port",
-        "sample": f"{sample} x 4096-point CFFT<float> forward, AoS, median of {len(times)} runs; "
-                  f"oracle FP32 restatement of the reference ({path} plan, switch order {sw})",
+        "sample": f"{sample} x 4096-point CFFT<float> forward, AoS, median of {nrun} runs on {t_all} threads; "
+                  f"scalar C restatement of the reference ({path} plan, switch order {sw}), -O3 -march=x86-64-v3",
         "transforms_per_s": round(sample / t, 1),
+        "t1": {"value": round(16 * N * 128 / t1 / 1e9, 3), "unit": "GB/s", "cores": 1,
+               "sample": f"128 transforms, median of {nrun1} runs", "transforms_per_s": round(128 / t1, 1)},
+        "cpu_model": cpus["model"],
+        "host_cpus": cpus,
     }
 
+
+# ---------------------------------------------------------------- MFMA variants
 
 MFMA_VARIANTS = (("f16_split", 2), ("bf16_split3", 5), ("f16", 1), ("bf16", 3), ("bf16_split", 4))
 
 
-def mfma_variants(x, y, xs, ref, stream, lib, args) -> dict:
-    """The same workload on the MFMA split-precision kernels: each radix-16
+def mfma_variants(x, y, stream, lib, args, plain_step) -> dict:
+    """The headline workload on the MFMA split-precision kernels (each radix-16
     pass as a 16x16 DFT-matrix product on FP16 / BF16 matrix cores with FP32
-    accumulate.  Throughput (event-timed like the main line) and accuracy vs
-    the FP64 oracle on the same sampled transforms."""
+    accumulate), timed interleaved with the plain FP32 kernel: rounds of
+    (plain, variant 1, ..., variant 5) windows with an event pair around every
+    launch, medians over at least MIN_TIMED launches each.  Accuracy vs the
+    FP64 oracle on two input sets: A, i.i.d. N(0, 1) (the timed batch, 16
+    sampled transforms), and B, the dynamic-range set (unit sinusoid, -60 dB
+    sinusoid, -100 dB white noise), 64 transforms through the same kernels."""
     import torch
+    import oracle
     import zldsp_fft_amd as zf
-    res = {}
-    idx = list(range(0, BATCH, BATCH // 16))
-    steps = args.steps
+    from zldsp_fft_amd import signals
     wanted = args.mfma_precisions.split(",") if args.mfma_precisions else None
-    for name, prec in MFMA_VARIANTS:
-        if wanted is not None and name not in wanted:
-            continue
-        plan = zf.CFFT(ORDER, prec)
-        h, sp = plan._h, stream.cuda_stream
+    variants = [(name, prec) for name, prec in MFMA_VARIANTS if wanted is None or name in wanted]
+    sp = stream.cuda_stream
+    plans, steps = {}, {"plain": plain_step}
+    for name, prec in variants:
+        plans[name] = zf.CFFT(ORDER, prec)
+        h = plans[name]._h
 
-        def step():
-            rc = lib.zlfft_cfft_exec(h, zf.FORWARD, zf.AOS, x.data_ptr(), None, zf.AOS, y.data_ptr(), None, BATCH, sp)
-            if rc:
-                raise RuntimeError(lib.zlfft_last_error().decode())
+        def step(h=h):
+            _call(lib, lib.zlfft_cfft_exec, h, zf.FORWARD, zf.AOS, x.data_ptr(), None, zf.AOS, y.data_ptr(), None,
+                  BATCH, sp)
+        steps[name] = step
+    order = ["plain"] + [n for n, _ in variants]
+    window = 10
+    rounds = max(10, -(-MIN_TIMED // window))
+    for name in order:
+        warm_up(torch, [steps[name]], seconds=0.1)
+    warm_up(torch, [steps[n] for n in order], seconds=WARM_S)
+    times = {n: [] for n in order}
+    for _ in range(rounds):
+        for name in order:
+            times[name] += time_each(torch, stream, [steps[name]], window)
+    # accuracy: set A from the timed batch, set B from a separate batch
+    idx = list(range(0, BATCH, BATCH // 16))
+    xa = x[idx].cpu().numpy().astype(np.complex128)
+    ref_a = np.stack([oracle.cfft(r, True, "f64") for r in xa])
+    nb = 64
+    xb = signals.dynamic_range_set(5, nb, N).astype(np.complex64)
+    ref_b = np.stack([oracle.cfft(r, True, "f64") for r in xb.astype(np.complex128)])
+    xb_d = torch.from_numpy(xb).to(x.device)
+    yb_d = torch.empty_like(xb_d)
 
-        for _ in range(max(20, args.warmup)):
-            step()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        a.record(stream)
-        for _ in range(steps):
-            step()
-        b.record(stream)
-        torch.cuda.synchronize()
-        ms = a.elapsed_time(b) / steps
-        ys = y[idx].cpu().numpy().astype(np.complex128)
-        rel = float((np.linalg.norm(ys - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
-        mbe = float((np.abs(ys - ref).max(axis=1) / np.abs(ref).max(axis=1)).max())
+    def errs(got, ref):
+        rel = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        mbe = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+        return float(rel.max()), float(rel.mean()), float(mbe.max())
+
+    res = {}
+    plain_ms = float(np.median(times["plain"]))
+    for name in order:
+        ms = float(np.median(times[name]))
         gbs = BYTES_PER_STEP / (ms * 1e-3) / 1e9
-        res[name] = {"GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "rel_l2_max": rel,
-                     "max_bin_err": mbe, "meets_1e-6": rel <= 1e-6}
-        plan.close()
+        d = {"GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4), "ms_median": round(ms, 5),
+             "ms_mean": round(float(np.mean(times[name])), 5), "launches": len(times[name])}
+        if name != "plain":
+            steps[name]()
+            torch.cuda.synchronize()
+            ra, rma, ma = errs(y[idx].cpu().numpy().astype(np.complex128), ref_a)
+            _call(lib, lib.zlfft_cfft_exec, plans[name]._h, zf.FORWARD, zf.AOS, xb_d.data_ptr(), None, zf.AOS,
+                  yb_d.data_ptr(), None, nb, sp)
+            torch.cuda.synchronize()
+            rb, rmb, mb = errs(yb_d.cpu().numpy().astype(np.complex128), ref_b)
+            d.update({"rel_l2_max": ra, "rel_l2_mean": rma, "max_bin_err": ma,
+                      "setB_rel_l2_max": rb, "setB_rel_l2_mean": rmb, "setB_max_bin_err": mb,
+                      "meets_1e-6": max(ra, rb) <= 1e-6, "vs_plain": round(plain_ms / ms, 4),
+                      "beats_plain": ms < plain_ms})
+            plans[name].close()
+        res["plain_f32" if name == "plain" else name] = d
+    res["timing"] = (f"interleaved: {rounds} rounds x {window} launches per kernel, event pair per launch, "
+                     f"median; warm-up {WARM_S} s")
     return res
 
+
+# ---------------------------------------------------------------- secondary lines
 
 def ragged_section(args) -> dict:
     """The ragged batch: 10,000 transforms with orders uniform in 6..14,
@@ -139,77 +306,61 @@ def ragged_section(args) -> dict:
     descs = workloads.ragged_mix(10000, 6)
     nbytes = sum(workloads.algorithmic_bytes(d) for d in descs)
     rb = zf.RaggedBatch(descs)
-    ms = workloads.time_ms(torch, rb.run, reps=max(20, args.steps // 4), warmup=10)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    lib = zf.load_library()
+    h = rb._h
+
+    def run():
+        _call(lib, lib.zlfft_ragged_exec, h, sp)
+    wall, kern = measure_line(torch, stream, [run])
     rb.close()
-    gbs = nbytes / (ms * 1e-3) / 1e9
-    return {"workload": "ragged_10000_orders6-14_mixed", "bytes_per_launch": nbytes, "ms_per_launch": round(ms, 5),
+    gbs = nbytes / (wall * 1e-3) / 1e9
+    kg = nbytes / (kern * 1e-3) / 1e9
+    return {"workload": "ragged_10000_orders6-14_mixed", "bytes_per_launch": nbytes, "ms_per_launch": round(wall, 5),
             "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
-            "transforms_per_s": round(len(descs) / (ms * 1e-3), 1)}
+            "kernel_ms_mean": round(kern, 5), "kernel_GB/s": round(kg, 1),
+            "transforms_per_s": round(len(descs) / (wall * 1e-3), 1),
+            "check": "every output vs the per-transform plain path <= 1e-6 rel-L2 in tests/test_gpu_ragged.py"}
 
 
-ROTATE = 5  # buffer sets per line: > 256 MiB in total, so the MALL cannot hold the working set
-
-
-def _timed(torch, stream, fns, reps, warmup=5):
-    """Mean ms per call over `reps` calls cycling through `fns`, HIP events on `stream`."""
-    for i in range(warmup):
-        fns[i % len(fns)]()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    a.record(stream)
-    for i in range(reps):
-        fns[i % len(fns)]()
-    b.record(stream)
-    torch.cuda.synchronize()
-    return a.elapsed_time(b) / reps
-
-
-def _line(nbytes, flops, count, ms, rel=None):
-    gbs = nbytes / (ms * 1e-3) / 1e9
-    d = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
-         "transforms_per_s": round(count / (ms * 1e-3), 1), "gflops": round(flops / (ms * 1e-3) / 1e9, 1)}
-    if rel is not None:
-        d["rel_l2_vs_fp64"] = rel
-    return d
-
-
-def config_lines(args) -> dict:
-    """Throughput of the analyzer batch (4096 RFFTs of 2048 reals: AoS bins and
-    squared magnitudes), of CFFT<double> (8192 x 4096 points) and of the
-    large SoA CFFTs (orders 16/18/20/22 at batch 64/16/4/1, forward and
-    backward).  Each line cycles through
-    ROTATE buffer sets; GB/s counts one read of the inputs and one write of
-    the outputs; GFLOP/s uses 5 N log2 N per CFFT and 2.5 N log2 N per RFFT.
-    rocFFT (through torch.fft) is timed beside it as an external reference
-    point on the AoS layouts it supports."""
+def analyzer_lines(args) -> dict:
+    """The analyzer batch (4096 RFFTs of 2048 Hann-windowed reals: AoS bins and
+    squared magnitudes), rocFFT's rfft beside it, and the STFT front-end."""
     import torch
     import oracle
     import zldsp_fft_amd as zf
     from zldsp_fft_amd import signals
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
-    reps = max(20, args.steps // 4)
+    sp = stream.cuda_stream
+    lib = zf.load_library()
     out = {}
-    # analyzer batch: 4096 Hann-windowed frames of 2048 reals
     n, frames = 2048, 4096
     nb = n // 2 + 1
     plan = zf.RFFT(11)
+    h = plan._h
     x0 = torch.from_numpy(signals.analyzer_frames(2, frames, n).astype(np.float32)).to(dev)
     xs = [x0] + [x0.clone() for _ in range(ROTATE - 1)]  # distinct addresses, same frames
     ys = [torch.empty(frames * nb, dtype=torch.complex64, device=dev) for _ in range(ROTATE)]
     ps = [torch.empty(frames * nb, dtype=torch.float32, device=dev) for _ in range(ROTATE)]
     flops = 2.5 * n * 11 * frames
-    ms = _timed(torch, stream, [lambda i=i: plan.forward(xs[i].view(-1), ys[i]) for i in range(ROTATE)], reps)
-    ref = oracle.rfft_forward(xs[0][7].cpu().numpy().astype(np.float64), "f64")
-    got = ys[0].view(frames, nb)[7].cpu().numpy()
-    out["rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, ms,
-                                       float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
-    ms = _timed(torch, stream, [lambda i=i: plan.forward_sqr_mag(xs[i].view(-1), ps[i]) for i in range(ROTATE)], reps)
-    got = ps[0].view(frames, nb)[7].cpu().numpy()
-    out["rfft2048_b4096_sqr_mag"] = _line(frames * (4 * n + 4 * nb), flops, frames, ms,
-                                           float(np.linalg.norm(got - np.abs(ref) ** 2) / np.linalg.norm(np.abs(ref) ** 2)))
-    ms = _timed(torch, stream, [lambda i=i: torch.fft.rfft(xs[i], dim=-1) for i in range(ROTATE)], reps)
-    out["rocfft_rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, ms)
+    fwd = [lambda i=i: _call(lib, lib.zlfft_rfft_forward_exec, h, xs[i].data_ptr(), zf.AOS, ys[i].data_ptr(), None,
+                             frames, sp) for i in range(ROTATE)]
+    wall, kern = measure_line(torch, stream, fwd)
+    ref = np.stack([oracle.rfft_forward(xs[0][f].cpu().numpy().astype(np.float64), "f64") for f in (7, 2048, 4095)])
+    got = ys[0].view(frames, nb)[[7, 2048, 4095]].cpu().numpy()
+    rel = float((np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)).max())
+    out["rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, wall, rel, kern)
+    sq = [lambda i=i: _call(lib, lib.zlfft_rfft_forward_sqr_mag_exec, h, xs[i].data_ptr(), ps[i].data_ptr(), frames,
+                            sp) for i in range(ROTATE)]
+    wall, kern = measure_line(torch, stream, sq)
+    got = ps[0].view(frames, nb)[[7, 2048, 4095]].cpu().numpy()
+    rsq = np.abs(ref) ** 2
+    rel = float((np.linalg.norm(got - rsq, axis=1) / np.linalg.norm(rsq, axis=1)).max())
+    out["rfft2048_b4096_sqr_mag"] = _line(frames * (4 * n + 4 * nb), flops, frames, wall, rel, kern)
+    wall, kern = measure_line(torch, stream, [lambda i=i: torch.fft.rfft(xs[i], dim=-1) for i in range(ROTATE)])
+    out["rocfft_rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, wall, None, kern)
     # STFT front-end: 4096 Hann-windowed frames of 2048 samples at hop 512 of
     # one signal, dB levels (frames cut and windowed at load time); bytes count
     # the frame samples read and the levels written
@@ -217,63 +368,102 @@ def config_lines(args) -> dict:
     sig = [torch.rand((frames - 1) * hop + n, device=dev) * 2 - 1 for _ in range(ROTATE)]
     win = torch.from_numpy(signals.hann(n).astype(np.float32)).to(dev)
     lv = [torch.empty(frames * nb, dtype=torch.float32, device=dev) for _ in range(ROTATE)]
-    ms = _timed(torch, stream, [lambda i=i: plan.stft(sig[i], hop, lv[i], window=win, output=zf.STFT_DB,
-                                                       db_floor=1e-12, nframes=frames) for i in range(ROTATE)], reps)
-    out["stft2048_hop512_b4096_hann_db"] = _line(frames * (4 * n + 4 * nb), flops, frames, ms)
-    del sig, lv
+    st = [lambda i=i: _call(lib, lib.zlfft_rfft_stft_exec, h, sig[i].data_ptr(), hop, win.data_ptr(), frames,
+                            zf.STFT_DB, zf.AOS, lv[i].data_ptr(), None, 1e-12, sp) for i in range(ROTATE)]
+    wall, kern = measure_line(torch, stream, st)
+    out["stft2048_hop512_b4096_hann_db"] = _line(frames * (4 * n + 4 * nb), flops, frames, wall, None, kern)
     plan.close()
-    del xs, ys, ps
+    return out
+
+
+def large_lines(args) -> dict:
+    """The large SoA CFFTs: orders 16/18/20/22 at batch 64/16/4/1 (4,194,304
+    points per call), forward and backward, through the C-ABI."""
+    import torch
+    import oracle
+    import zldsp_fft_amd as zf
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+    lib = zf.load_library()
+    out = {}
+    for order, batch in ((16, 64), (18, 16), (20, 4), (22, 1)):
+        n = 1 << order
+        plan = zf.CFFT(order)
+        h = plan._h
+        g = torch.Generator(device=dev).manual_seed(3)
+        ins = [(torch.randn(batch * n, device=dev, generator=g), torch.randn(batch * n, device=dev, generator=g))
+               for _ in range(ROTATE)]
+        outs = [(torch.empty(batch * n, device=dev), torch.empty(batch * n, device=dev)) for _ in range(ROTATE)]
+        for name, d in (("forward", zf.FORWARD), ("backward", zf.BACKWARD)):
+            fns = [lambda i=i, d=d: _call(lib, lib.zlfft_cfft_exec, h, d, zf.SOA, ins[i][0].data_ptr(),
+                                          ins[i][1].data_ptr(), zf.SOA, outs[i][0].data_ptr(), outs[i][1].data_ptr(),
+                                          batch, sp) for i in range(ROTATE)]
+            wall, kern = measure_line(torch, stream, fns)
+            x = ins[0][0][:n].cpu().numpy().astype(np.float64) + 1j * ins[0][1][:n].cpu().numpy()
+            ref = oracle.cfft(x, name == "forward", "f64")
+            got = outs[0][0][:n].cpu().numpy() + 1j * outs[0][1][:n].cpu().numpy().astype(np.float64)
+            out[f"cfft_o{order}_b{batch}_soa_{name}"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall,
+                                                              float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+                                                              kern)
+        plan.close()
+        del ins, outs
+        torch.cuda.empty_cache()
+    return out
+
+
+def config_lines(args) -> dict:
+    """Secondary lines: the analyzer batch and STFT, CFFT<double>
+    (8192 x 4096 points), the headline batch from host memory (PCIe-inclusive,
+    never `value`), and the large SoA CFFTs.  GB/s counts one read of the
+    inputs and one write of the outputs; GFLOP/s uses 5 N log2 N per CFFT and
+    2.5 N log2 N per RFFT.  rocFFT (through torch.fft) is timed beside the
+    analyzer batch as an external point of reference."""
+    import torch
+    import oracle
+    import zldsp_fft_amd as zf
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+    lib = zf.load_library()
+    out = analyzer_lines(args)
     torch.cuda.empty_cache()
     # CFFT<double>: 8192 x 4096 points AoS (1 GiB of traffic per batch, like the headline batch)
     n64, b64 = 4096, 8192
     plan = zf.CFFT(12, zf.PREC_F64)
+    h = plan._h
     g = torch.Generator(device=dev).manual_seed(7)
     ins = [torch.randn((b64, n64), dtype=torch.complex128, device=dev, generator=g) for _ in range(2)]
     outs = [torch.empty_like(ins[0]) for _ in range(2)]
-    ms = _timed(torch, stream, [lambda i=i: plan.forward(ins[i], outs[i]) for i in range(2)], reps)
+    fns = [lambda i=i: _call(lib, lib.zlfft_cfft_exec, h, zf.FORWARD, zf.AOS, ins[i].data_ptr(), None, zf.AOS,
+                             outs[i].data_ptr(), None, b64, sp) for i in range(2)]
+    wall, kern = measure_line(torch, stream, fns, reps=40)
     ref = oracle.cfft(ins[0][5].cpu().numpy(), True, "f64")
     got = outs[0][5].cpu().numpy()
-    out["cfft64_o12_b8192_aos_forward"] = _line(32 * n64 * b64, 5.0 * n64 * 12 * b64, b64, ms,
-                                                 float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    out["cfft64_o12_b8192_aos_forward"] = _line(32 * n64 * b64, 5.0 * n64 * 12 * b64, b64, wall,
+                                                 float(np.linalg.norm(got - ref) / np.linalg.norm(ref)), kern)
     plan.close()
     del ins, outs
     torch.cuda.empty_cache()
     # host buffers (PCIe-inclusive): the headline batch from host memory through
     # the pipelined host path, pageable and pinned, wall-clock per call
-    import time as _time
     xh = np.random.default_rng(1).standard_normal((BATCH, 2 * N)).astype(np.float32).view(np.complex64)
     yh = np.empty_like(xh)
     plan = zf.CFFT(ORDER)
     for name, (src, dst) in (("pageable", (xh, yh)),
-                             ("pinned", (torch.from_numpy(xh).pin_memory(), torch.empty(xh.shape, dtype=torch.complex64).pin_memory()))):
+                             ("pinned", (torch.from_numpy(xh).pin_memory(),
+                                         torch.empty(xh.shape, dtype=torch.complex64).pin_memory()))):
         plan.forward(src, dst)  # warm-up: staging allocation
         t = []
         for _ in range(3):
-            t0 = _time.perf_counter()
+            t0 = time.perf_counter()
             plan.forward(src, dst)
-            t.append(_time.perf_counter() - t0)
+            t.append(time.perf_counter() - t0)
         ms = 1e3 * float(np.median(t))
         out[f"host_{name}_cfft4096_b16384"] = _line(BYTES_PER_STEP, 5.0 * N * ORDER * BATCH, BATCH, ms)
     plan.close()
     del xh, yh
-    # large SoA CFFTs: 4,194,304 points per call
-    for order, batch in ((16, 64), (18, 16), (20, 4), (22, 1)):
-        n = 1 << order
-        plan = zf.CFFT(order)
-        g = torch.Generator(device=dev).manual_seed(3)
-        ins = [(torch.randn(batch * n, device=dev, generator=g), torch.randn(batch * n, device=dev, generator=g))
-               for _ in range(ROTATE)]
-        outs = [(torch.empty(batch * n, device=dev), torch.empty(batch * n, device=dev)) for _ in range(ROTATE)]
-        for name, fn in (("forward", plan.forward), ("backward", plan.backward)):
-            ms = _timed(torch, stream, [lambda i=i, fn=fn: fn(ins[i], outs[i]) for i in range(ROTATE)], reps)
-            x = ins[0][0][:n].cpu().numpy().astype(np.float64) + 1j * ins[0][1][:n].cpu().numpy()
-            ref = oracle.cfft(x, name == "forward", "f64")
-            got = outs[0][0][:n].cpu().numpy() + 1j * outs[0][1][:n].cpu().numpy().astype(np.float64)
-            out[f"cfft_o{order}_b{batch}_soa_{name}"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, ms,
-                                                              float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
-        plan.close()
-        del ins, outs
-        torch.cuda.empty_cache()
+    out.update(large_lines(args))
     return out
 
 
@@ -281,8 +471,8 @@ def rocfft_c4(x, stream, args) -> dict:
     """rocFFT (through torch.fft) on the same 16384 x 4096 AoS batch: an
     external reference point, timed the same way."""
     import torch
-    ms = _timed(torch, stream, [lambda: torch.fft.fft(x, dim=-1)], max(20, args.steps // 4), warmup=10)
-    return _line(BYTES_PER_STEP, 5.0 * N * ORDER * BATCH, BATCH, ms)
+    wall, kern = measure_line(torch, stream, [lambda: torch.fft.fft(x, dim=-1)], reps=40)
+    return _line(BYTES_PER_STEP, 5.0 * N * ORDER * BATCH, BATCH, wall, None, kern)
 
 
 def traffic_bytes():
@@ -312,10 +502,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    if args.only in ("configs", "ragged"):
-        out = {"section": args.only}
-        out[args.only] = config_lines(args) if args.only == "configs" else ragged_section(args)
-        print(json.dumps(out), flush=True)
+    sections = {"configs": config_lines, "ragged": ragged_section, "large": large_lines, "analyzer": analyzer_lines}
+    if args.only in sections:
+        print(json.dumps({"section": args.only, args.only: sections[args.only](args)}), flush=True)
         return
     dev = torch.device("cuda:0")
     plan = zf.CFFT(ORDER)
@@ -327,9 +516,7 @@ def main():
     xp, yp, h, sp = x.data_ptr(), y.data_ptr(), plan._h, stream.cuda_stream
 
     def step():
-        rc = lib.zlfft_cfft_exec(h, zf.FORWARD, zf.AOS, xp, None, zf.AOS, yp, None, BATCH, sp)
-        if rc:
-            raise RuntimeError(lib.zlfft_last_error().decode())
+        _call(lib, lib.zlfft_cfft_exec, h, zf.FORWARD, zf.AOS, xp, None, zf.AOS, yp, None, BATCH, sp)
 
     if args.copy_probe:
         # known-size copy: reads and writes BATCH * N * 8 bytes, for scaling HBM counters
@@ -337,20 +524,16 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
     if args.only == "mfma":
-        idx = list(range(0, BATCH, BATCH // 16))
-        xs = x[idx].cpu().numpy().astype(np.complex128)
-        ref = np.stack([oracle.cfft(r, True, "f64") for r in xs])
-        print(json.dumps({"section": "mfma", "mfma": mfma_variants(x, y, xs, ref, stream, lib, args)}), flush=True)
+        print(json.dumps({"section": "mfma", "mfma": mfma_variants(x, y, stream, lib, args, step)}), flush=True)
         return
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
 
-    # timed region: K steps, each launch bracketed by events on the launch stream
+    # warm-up: --warmup steps, extended by time so the clocks settle; no
+    # synchronisation before the timed region
+    warm_up(torch, [step], seconds=WARM_S, min_calls=args.warmup)
+    # timed region: exactly --steps steps, each launch bracketed by events on the launch stream
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     t_start.record(stream)
     for i in range(args.steps):
         ev[i][0].record(stream)
@@ -393,17 +576,20 @@ def main():
         "transforms_per_s": round(BATCH / (ms_per_step * 1e-3), 1),
         "gflops_5nlog2n": round(5 * N * ORDER * BATCH / (ms_per_step * 1e-3) / 1e9, 1),
         "kernel_ms_avg": round(kern_ms, 5),
+        "kernel_ms_median": round(float(np.median(launch_ms)), 5),
+        "warmup_rule": f"{args.warmup} steps, extended to >= {WARM_S} s of back-to-back launches, no sync gap",
     }
     if args.only == "main":
         args.no_mfma = args.no_configs = args.no_ragged = args.no_cpu = True
     if not args.no_mfma:
-        out["mfma"] = mfma_variants(x, y, xs, ref, stream, lib, args)
+        out["mfma"] = mfma_variants(x, y, stream, lib, args, step)
     if args.only != "main":
         out["rocfft"] = rocfft_c4(x, stream, args)
     if not args.no_configs:
+        del x, y
+        torch.cuda.empty_cache()
         out["configs"] = config_lines(args)
     if not args.no_ragged:
-        del x, y
         torch.cuda.empty_cache()
         out["ragged"] = ragged_section(args)
     if not args.no_cpu:

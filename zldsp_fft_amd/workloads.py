@@ -8,52 +8,65 @@ import zldsp_fft_amd as zf
 from zldsp_fft_amd import signals
 
 
-def _rand(torch, g, shape):
-    return torch.rand(shape, generator=g, device="cuda", dtype=torch.float32) * 2 - 1
+_ALIGN = 64  # floats: every buffer of a slab starts on a 256-byte boundary
 
 
-def ragged_desc(torch, g, kind, order, direction, layout):
-    """One descriptor (kind, order, direction, layout, in, out) with random
-    U[-1, 1) input and an empty output, shaped as the CFFT / RFFT call takes them."""
-    n = 1 << order
-    nb = n // 2 + 1
-    if kind == zf.KIND_CFFT:
-        if layout == zf.AOS:
-            inb = torch.view_as_complex(_rand(torch, g, (n, 2)).contiguous())
-            outb = torch.empty(n, dtype=torch.complex64, device="cuda")
-        else:
-            inb = (_rand(torch, g, n), _rand(torch, g, n))
-            outb = (torch.empty(n, device="cuda"), torch.empty(n, device="cuda"))
-    elif direction in (zf.FORWARD, zf.FORWARD_SQR_MAG):
-        inb = _rand(torch, g, n)
-        if direction == zf.FORWARD_SQR_MAG:
-            outb = torch.empty(nb, device="cuda")
-        elif layout == zf.AOS:
-            outb = torch.empty(nb, dtype=torch.complex64, device="cuda")
-        else:
-            outb = (torch.empty(nb, device="cuda"), torch.empty(nb, device="cuda"))
-    else:
-        inb = (torch.view_as_complex(_rand(torch, g, (nb, 2)).contiguous()) if layout == zf.AOS
-               else (_rand(torch, g, nb), _rand(torch, g, nb)))
-        outb = torch.empty(n, device="cuda")
-    return (kind, order, direction, layout, inb, outb)
+def _slab_views(torch, slab, sizes):
+    """Consecutive 256-byte-aligned float views of a 1-D tensor, one per size."""
+    out, off = [], 0
+    for n in sizes:
+        out.append(slab[off:off + n])
+        off += (n + _ALIGN - 1) // _ALIGN * _ALIGN
+    return out
+
+
+def _slab_len(sizes) -> int:
+    return sum((n + _ALIGN - 1) // _ALIGN * _ALIGN for n in sizes)
 
 
 def ragged_mix(count: int, seed: int, orders=(6, 14)):
     """The mixed ragged set: orders uniform in [lo, hi], CFFT / RFFT, forward /
-    backward and AoS / SoA each 50/50 (SplitMix64 choices), inputs U[-1, 1)."""
+    backward and AoS / SoA each 50/50 (SplitMix64 choices), inputs U[-1, 1).
+
+    Returns descriptors (kind, order, direction, layout, in, out) shaped as the
+    CFFT / RFFT calls take them, whose buffers are views into one input slab
+    (filled by a single device RNG call) and one output slab, so building the
+    set costs three device allocations, not one per transform."""
     import torch
     u = signals.uniform(seed, 4 * count, 0.0, 1.0).reshape(count, 4)
     lo, hi = orders
-    g = torch.Generator(device="cuda").manual_seed(seed)
-    out = []
+    shapes = []  # per descriptor: (kind, order, direction, layout, in float counts, out float counts)
     for i in range(count):
         order = lo + min(hi - lo, int(u[i, 0] * (hi - lo + 1)))
         kind = zf.KIND_CFFT if u[i, 1] < 0.5 else zf.KIND_RFFT
         direction = zf.FORWARD if u[i, 2] < 0.5 else zf.BACKWARD
         layout = zf.AOS if u[i, 3] < 0.5 else zf.SOA
-        out.append(ragged_desc(torch, g, kind, order, direction, layout))
-    return out
+        fi, ki = _side_floats(kind, direction, layout, order, 0)
+        fo, ko = _side_floats(kind, direction, layout, order, 1)
+        shapes.append((kind, order, direction, layout, [fi] * ki, [fo] * ko))
+    in_sizes = [n for s in shapes for n in s[4]]
+    out_sizes = [n for s in shapes for n in s[5]]
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    in_slab = torch.rand(_slab_len(in_sizes), generator=g, device="cuda")
+    in_slab.mul_(2).sub_(1)
+    out_slab = torch.empty(_slab_len(out_sizes), device="cuda")
+    ins = iter(_slab_views(torch, in_slab, in_sizes))
+    outs = iter(_slab_views(torch, out_slab, out_sizes))
+    descs = []
+    for kind, order, direction, layout, fin, fout in shapes:
+        complex_in = kind == zf.KIND_CFFT or direction == zf.BACKWARD
+        complex_out = kind == zf.KIND_CFFT or direction == zf.FORWARD
+
+        def side(views, cplx):
+            if len(views) == 2:
+                return tuple(views)
+            v = views[0]
+            return torch.view_as_complex(v.view(-1, 2)) if cplx else v
+
+        inb = side([next(ins) for _ in fin], complex_in)
+        outb = side([next(outs) for _ in fout], complex_out)
+        descs.append((kind, order, direction, layout, inb, outb))
+    return descs
 
 
 VARIANTS = ([(zf.KIND_CFFT, d, l) for d in (zf.FORWARD, zf.BACKWARD) for l in (zf.AOS, zf.SOA)]
