@@ -57,7 +57,7 @@ def test_rfft_host_batches():
     idx = [0, 777, 4999]
     ref = np.stack([oracle.rfft_forward(x[i].astype(np.float64), "f64") for i in idx])
     assert signals.rel_l2(bins[idx].astype(np.complex128), ref).max() <= TOL
-    assert signals.rel_l2(pw[idx].astype(np.float64), np.abs(ref) ** 2).max() <= 2 * TOL
+    assert signals.rel_l2(pw[idx].astype(np.float64), np.abs(ref) ** 2).max() <= TOL
     back = np.zeros_like(x)
     plan.backward(bins, back)
     assert signals.rel_l2(back[idx].astype(np.float64), (n / 2) * x[idx].astype(np.float64)).max() <= TOL
@@ -80,3 +80,40 @@ def test_host_batches_multipass_and_f64():
     y64 = np.zeros_like(x64)
     p64.forward(x64, y64)
     assert signals.rel_l2(y64, np.fft.fft(x64, axis=1)).max() <= 1e-13
+
+
+def test_device_input_still_being_produced():
+    """A device input whose producing kernels are still queued on the legacy
+    default stream (torch's default stream), passed straight to the C-ABI with
+    a host output: the library's streams wait for that work before reading,
+    so the result is the transform of the finished input.  The same through
+    the Python wrapper on a non-default torch stream, which it synchronises."""
+    import ctypes
+    order, batch = 12, 2048
+    n = 1 << order
+    lib = zf.load_library()
+    plan = zf.CFFT(order)
+    g = torch.Generator(device="cuda").manual_seed(21)
+    base = torch.randn((batch, n), dtype=torch.complex64, device="cuda", generator=g)
+    torch.cuda.synchronize()
+    x = base
+    for _ in range(40):  # a few ms of queued work producing x
+        x = x * 1.0001 + 0.5
+    y = np.zeros((batch, n), np.complex64)
+    rc = lib.zlfft_cfft_execute_batch(plan._h, zf.FORWARD, zf.AOS, ctypes.c_void_p(x.data_ptr()), None, zf.AOS,
+                                      y.ctypes.data, None, batch)
+    assert rc == 0, lib.zlfft_last_error()
+    idx = [0, 1000, batch - 1]
+    xs = x[idx].cpu().numpy().astype(np.complex128)
+    ref = np.stack([oracle.cfft(r, True, "f64") for r in xs])
+    assert signals.rel_l2(y[idx].astype(np.complex128), ref).max() <= TOL
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        x2 = base
+        for _ in range(40):
+            x2 = x2 * 0.9999 - 0.25
+        y2 = np.zeros((batch, n), np.complex64)
+        plan.forward(x2, y2)
+    xs2 = x2[idx].cpu().numpy().astype(np.complex128)
+    ref2 = np.stack([oracle.cfft(r, True, "f64") for r in xs2])
+    assert signals.rel_l2(y2[idx].astype(np.complex128), ref2).max() <= TOL

@@ -100,3 +100,52 @@ def test_mfma_persistent_form_matches(name):
         ref = np.stack([oracle.cfft(x[i].astype(np.complex128), forward, "f64") for i in idx])
         got = y[idx].cpu().numpy().astype(np.complex128)
         assert signals.rel_l2(got, ref).max() <= tol, (name, forward)
+
+
+def _dynamic_range_gpu(seed, batch, n):
+    """The dynamic-range set, generated on the device: per transform a unit
+    complex exponential at frequency f1, a -60 dB one at f2 (f1, f2 ~ U[0, N)
+    bins, SplitMix64) and complex white noise of sigma 1e-5 per component."""
+    p = torch.from_numpy(signals.uniform(seed, batch * 2, 0.0, 1.0).reshape(batch, 2)).cuda()
+    t = torch.arange(n, dtype=torch.float64, device="cuda")
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    noise = 1e-5 * torch.randn((batch, n), dtype=torch.complex128, device="cuda", generator=g) * np.sqrt(2.0)
+    ph1 = 2 * np.pi * (p[:, 0:1] * n) * t / n
+    ph2 = 2 * np.pi * (p[:, 1:2] * n) * t / n
+    x = torch.polar(torch.ones_like(ph1), ph1) + 1e-3 * torch.polar(torch.ones_like(ph2), ph2) + noise
+    return x.to(torch.complex64)
+
+
+@pytest.mark.parametrize("name", ["f16_split", "bf16_split3"])
+def test_mfma_full_batch_both_sets(name):
+    """The full 16384 x 4096-point batch of the headline workload on the split
+    MFMA kernels (the persistent form), both input sets, both directions:
+    every transform within rel-L2 1e-6 of the FP64 GPU path (CFFT<double>,
+    itself within 5e-16 of the FP64 oracle), and sampled transforms against
+    the oracle directly."""
+    prec, tol = PRECS[name]
+    batch, n = 16384, 4096
+    plan, p64 = zf.CFFT(12, prec), zf.CFFT(12, zf.PREC_F64)
+    g = torch.Generator(device="cuda").manual_seed(4)
+    sets = {"normal": torch.randn((batch, n), dtype=torch.complex64, device="cuda", generator=g),
+            "dynamic": _dynamic_range_gpu(5, batch, n)}
+    for sname, x in sets.items():
+        x64 = x.to(torch.complex128)
+        for forward in (True, False):
+            y = torch.empty_like(x)
+            (plan.forward if forward else plan.backward)(x, y)
+            y64 = torch.empty_like(x64)
+            (p64.forward if forward else p64.backward)(x64, y64)
+            d = y.to(torch.complex128) - y64
+            rl2 = (torch.linalg.vector_norm(d, dim=1) / torch.linalg.vector_norm(y64, dim=1)).max().item()
+            mbe = (d.abs().amax(dim=1) / y64.abs().amax(dim=1)).max().item()
+            print(f"mfma {name} full batch set={sname} fwd={forward}: relL2 max={rl2:.3e} max-bin={mbe:.3e}",
+                  flush=True)
+            assert torch.isfinite(y).all().item()
+            assert rl2 <= tol
+            idx = [0, 8191, 16383]
+            ref = np.stack([oracle.cfft(r, forward, "f64") for r in x[idx].cpu().numpy().astype(np.complex128)])
+            assert signals.rel_l2(y[idx].cpu().numpy().astype(np.complex128), ref).max() <= tol
+            del y, y64, d
+        del x64
+    torch.cuda.empty_cache()

@@ -221,6 +221,18 @@ def _stream_of(*tensors) -> int:
     return torch.cuda.current_stream(dev).cuda_stream
 
 
+def _after_torch(*bufs) -> None:
+    """Before a synchronous call that mixes host and device buffers: wait for
+    torch's current stream on every device a device buffer lives on, so a
+    kernel still producing an input (or still reading an output) on that
+    stream has finished.  (The library itself waits for the legacy default
+    stream; torch's current stream may be another one.)"""
+    import torch
+    devs = {b.device for b in bufs if _is_dev(b)}
+    for d in devs:
+        torch.cuda.current_stream(d).synchronize()
+
+
 def _batch(n_have: int, n_per: int, name: str) -> int:
     if n_per == 0 or n_have % n_per != 0 or n_have == 0:
         raise ValueError(f"{name}: holds {n_have} elements, not a positive multiple of {n_per}")
@@ -292,6 +304,7 @@ class CFFT(_Plan):
         b = _batch(si.ncomplex, n, "in_buffer")
         if _batch(so.ncomplex, n, "out_buffer") != b:
             raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _after_torch(*si.ref, *so.ref)
         _check(self._lib.zlfft_cfft_execute_batch(self._h, direction, si.layout, si.p0, si.p1, so.layout, so.p0, so.p1,
                                                   b))
 
@@ -327,6 +340,7 @@ class RFFT(_Plan):
         b = _batch(_nbytes(x) // self._ss, n, "in_buffer")
         if _batch(so.ncomplex, nb, "out_buffer") != b:
             raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _after_torch(x, *so.ref)
         _check(self._lib.zlfft_rfft_forward_batch(self._h, _ptr(x), so.layout, so.p0, so.p1, b))
 
     def backward(self, in_buffer, out_buffer) -> None:
@@ -345,6 +359,7 @@ class RFFT(_Plan):
         b = _batch(si.ncomplex, nb, "in_buffer")
         if _batch(_nbytes(y) // self._ss, n, "out_buffer") != b:
             raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _after_torch(*si.ref, y)
         _check(self._lib.zlfft_rfft_backward_batch(self._h, si.layout, si.p0, si.p1, _ptr(y), b))
 
     def forward_sqr_mag(self, in_buffer, out_buffer) -> None:
@@ -362,6 +377,7 @@ class RFFT(_Plan):
         b = _batch(_nbytes(x) // self._ss, n, "in_buffer")
         if _batch(_nbytes(y) // self._ss, nb, "out_buffer") != b:
             raise ValueError("in_buffer and out_buffer hold different batch sizes")
+        _after_torch(x, y)
         _check(self._lib.zlfft_rfft_forward_sqr_mag_batch(self._h, _ptr(x), _ptr(y), b))
 
     def stft(self, signal, hop: int, out_buffer, window=None, output: int = STFT_BINS, db_floor: float = 1e-20,

@@ -250,6 +250,14 @@ struct zlfft_plan_s {
     void* d_twb[4] = {};
     void* d_work = nullptr;     // intermediate buffers, grown on demand
     size_t work_bytes = 0;
+    // the workspace is used on one stream at a time: a launch on another
+    // stream than the last one waits for the last one's work (work_ev)
+    hipEvent_t work_ev = nullptr;
+    hipStream_t work_stream = nullptr;
+    bool work_ev_set = false;
+    // the synchronous calls on host or device buffers first wait for the work
+    // queued on the legacy default stream (caller_ev recorded there)
+    hipEvent_t caller_ev = nullptr;
     // calls on host pointers: a pipeline of slots (see run_host)
     struct HostSlot {
         void* pin = nullptr;        // pinned host staging of one chunk
@@ -270,12 +278,26 @@ int check_plan(zlfft_plan p) {
     return ZLFFT_OK;
 }
 
+bool capturing(void* stream) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(reinterpret_cast<hipStream_t>(stream), &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+// waits on the host for the last queued work on the workspace
+int wait_work(zlfft_plan p) {
+    if (p->work_ev_set) ZL_HIP(hipEventSynchronize(p->work_ev));
+    return ZLFFT_OK;
+}
+
 int ensure_work(zlfft_plan p, size_t bytes) {
     if (p->work_bytes >= bytes) return ZLFFT_OK;
     if (p->d_work) {
-        // the previous workspace may still be in use by queued work on any stream
-        hipError_t e = hipDeviceSynchronize();
-        if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+        // the previous workspace may still be in use by queued work
+        if (int rc = wait_work(p)) return rc;
         (void)hipFree(p->d_work);
         p->d_work = nullptr;
         p->work_bytes = 0;
@@ -373,11 +395,12 @@ int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, s
     return ZLFFT_OK;
 }
 
-// user_in: in0 / in1 is the caller's input (0: frames the STFT just wrote)
-int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
-                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f, int user_in = 1) {
-    if (batch == 0) return ZLFFT_OK;
-    if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
+// user_in: in0 / in1 is the caller's input (0: frames the STFT just wrote).
+// The workspace serves one stream at a time: a call on another stream than the previous call's first waits for
+// that call's work (not inside a stream capture, where the caller orders the
+// captured work).
+int launch_macro_body(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0,
+                      void* out1, size_t batch, int backward, void* stream, int db, float db_floor, int user_in) {
     void* work = p->d_work;
     if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work, 3);
     const size_t n = (size_t)1 << p->corder;
@@ -393,6 +416,33 @@ int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, con
                         : rfft_pre<float2>(p, lin, in0, in1, z, batch, stream))
         return rc;
     return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work, 2);
+}
+
+// orders `stream` after the previous workspace user (see launch_macro_body)
+int work_acquire(zlfft_plan p, void* stream) {
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (p->work_ev_set && p->work_stream != s && !capturing(stream)) ZL_HIP(hipStreamWaitEvent(s, p->work_ev, 0));
+    return ZLFFT_OK;
+}
+
+int work_release(zlfft_plan p, void* stream) {
+    if (capturing(stream)) return ZLFFT_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ZL_HIP(hipEventRecord(p->work_ev, s));
+    p->work_stream = s;
+    p->work_ev_set = true;
+    return ZLFFT_OK;
+}
+
+int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
+                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f, int user_in = 1) {
+    if (batch == 0) return ZLFFT_OK;
+    if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
+    if (int rc = work_acquire(p, stream)) return rc;
+    const int rc = launch_macro_body(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream, db, db_floor,
+                                     user_in);
+    if (rc) return rc;
+    return work_release(p, stream);
 }
 
 int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, size_t batch, int backward,
@@ -600,12 +650,23 @@ int run_host(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
 template <class Body>
 int run_host_chunks(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
     MemKind kin[2] = {MEM_DEVICE, MEM_DEVICE}, kout[2] = {MEM_DEVICE, MEM_DEVICE};
-    bool all_dev = true;
+    bool all_dev = true, any_dev = false;
     for (int k = 0; k < 2; ++k) {
         if (in.p[k]) all_dev &= (kin[k] = mem_kind(in.p[k])) == MEM_DEVICE;
         if (out.p[k]) all_dev &= (kout[k] = mem_kind(out.p[k])) == MEM_DEVICE;
+        any_dev |= (in.p[k] && kin[k] == MEM_DEVICE) || (out.p[k] && kout[k] == MEM_DEVICE);
     }
+    // device buffers of the caller: the library's streams do not synchronise
+    // with the legacy default stream, so every stream that touches them first
+    // waits for the work queued there (a kernel still producing the input, or
+    // still reading the output's old contents)
+    if (any_dev) ZL_HIP(hipEventRecord(p->caller_ev, nullptr));
+    auto after_caller = [&](hipStream_t s) -> int {
+        if (any_dev) ZL_HIP(hipStreamWaitEvent(s, p->caller_ev, 0));
+        return ZLFFT_OK;
+    };
     if (all_dev) {
+        if (int rc = after_caller(p->stream)) return rc;
         int rc = body(in.p[0], in.p[1], const_cast<void*>(out.p[0]), const_cast<void*>(out.p[1]), batch, p->stream);
         if (rc) return rc;
         ZL_HIP(hipStreamSynchronize(p->stream));
@@ -622,8 +683,11 @@ int run_host_chunks(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
         o += round_up(chunk * bsz[k]);
     }
     const int nslots = p->macro ? 1 : shape.slots;
-    for (int i = 0; i < nslots; ++i)
+    for (int i = 0; i < nslots; ++i) {
         if (int rc = ensure_slot(p, i, o)) return rc;
+        if (int rc = after_caller(p->slot[i].stream)) return rc;
+        if (int rc = after_caller(p->slot[i].stream_out)) return rc;
+    }
     size_t pend_first[kMaxSlots] = {}, pend_n[kMaxSlots] = {};
     // waits for a slot's chunk and moves its outputs from pinned staging to pageable user memory
     auto finish = [&](int i) -> int {
@@ -753,6 +817,12 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hip_fail(e, "hipStreamCreateWithFlags"));
 
+    e = hipEventCreateWithFlags(&p->work_ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->caller_ev, hipEventDisableTiming);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipEventCreateWithFlags"));
+    e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceGetAttribute"));
+
     if (p->macro) {
         const zl::MacroPlan mpl = f64 ? zl::macro_plan64((int)corder) : zl::macro_plan((int)corder);
         p->mp = mpl.np;
@@ -788,8 +858,6 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         if (e != hipSuccess) return cleanup(hip_fail(e, "twiddle generation"));
     }
     if (!p->macro) {
-        e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
-        if (e != hipSuccess) return cleanup(hip_fail(e, "hipDeviceGetAttribute"));
         for (int k = 0; k < 6; ++k)
             for (int i = 0; i < 2; ++i)
                 for (int o = 0; o < 2; ++o) {
@@ -874,6 +942,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
 int zlfft_plan_destroy(zlfft_plan p) {
     if (!p) return fail(ZLFFT_ERR_NULL, "plan is NULL");
     if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->work_ev_set) (void)hipEventSynchronize(p->work_ev);  // multi-pass work queued on a caller's stream
     if (p->d_tw) (void)hipFree(p->d_tw);
     if (p->d_img) (void)hipFree(p->d_img);
     if (p->d_sink) (void)hipFree(p->d_sink);
@@ -890,6 +959,8 @@ int zlfft_plan_destroy(zlfft_plan p) {
         if (sl.stream_out) (void)hipStreamDestroy(sl.stream_out);
     }
     if (p->d_work) (void)hipFree(p->d_work);
+    if (p->work_ev) (void)hipEventDestroy(p->work_ev);
+    if (p->caller_ev) (void)hipEventDestroy(p->caller_ev);
     for (int q = 0; q < 4; ++q) {
         if (p->d_itw[q]) (void)hipFree(p->d_itw[q]);
         if (p->d_twa[q]) (void)hipFree(p->d_twa[q]);
@@ -1071,6 +1142,7 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
     const size_t n = (size_t)1 << p->order;
     const size_t mw = macro_work_bytes(p, nframes);
     if (int rc = ensure_work(p, mw + nframes * n * p->ss)) return rc;
+    if (int rc = work_acquire(p, stream)) return rc;
     void* frames = static_cast<char*>(p->d_work) + mw;
     const long long total = (long long)(nframes * n);
     const unsigned grid = (unsigned)((total + 255) / 256);
@@ -1084,7 +1156,9 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
                                                      static_cast<float*>(frames), (long long)hop, (long long)n,
                                                      (long long)nframes);
     ZL_HIP(hipGetLastError());
-    return launch_macro(p, kind, 0, lout, frames, nullptr, out0, out1, nframes, 0, stream, db, db_floor, 0);
+    if (int rc = launch_macro_body(p, kind, 0, lout, frames, nullptr, out0, out1, nframes, 0, stream, db, db_floor, 0))
+        return rc;
+    return work_release(p, stream);
 }
 
 int zlfft_plan_copy_rfft_twiddles(zlfft_plan p, float* host_out, size_t n_floats) {
