@@ -38,6 +38,12 @@
  * Threading: a plan may be used by one thread at a time (the reference's
  * instances are not shareable either, src/common/zldsp_fft_common_init.hpp:24-43);
  * distinct plans are independent.
+ * Streams: the multi-pass workspace of a plan is handed from stream to stream:
+ * a call on another stream than the plan's previous multi-pass call waits
+ * (on the device) for that call's work, except inside a stream capture,
+ * where the caller orders the captured work.  The synchronous and *_batch
+ * calls wait for the work queued on the legacy default stream before they
+ * touch device buffers.
  */
 #ifndef ZLFFT_H
 #define ZLFFT_H
@@ -120,6 +126,22 @@ size_t zlfft_plan_get_order(zlfft_plan plan);
  * in1 / out1 are the imaginary arrays for ZLFFT_SOA and ignored for AoS. */
 int zlfft_cfft_exec(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
                     int out_layout, void *out0, void *out1, size_t batch, void *stream);
+
+/* As zlfft_cfft_exec, allocating nothing.  No counterpart in the reference:
+ * for callers that cannot call hipMalloc (code under hipGraph stream capture,
+ * serving loops, harnesses that forbid allocation).  Multi-pass plans (CFFT
+ * orders above 14) keep their intermediates in a library-owned device array
+ * of at most 64 MiB per GPU (allocated with the library's code object, held
+ * until the process ends) instead of the plan workspace: a call whose
+ * intermediates exceed it (FP32: 2 * batch * 2^order * 8 bytes for orders
+ * 21-28, half that for orders 15-20; FP64 twice as much) returns
+ * ZLFFT_ERR_INVALID_ARG.  That array is shared by every call on the GPU, so
+ * calls that overlap on one GPU (other streams, other plans, other threads)
+ * race on it; calls queued on one stream do not.  One-kernel plans (orders up
+ * to 14) need no scratch and run exactly as zlfft_cfft_exec.  The plan's
+ * device must be the calling thread's current device. */
+int zlfft_cfft_exec_noalloc(zlfft_plan plan, int direction, int in_layout, const void *in0, const void *in1,
+                            int out_layout, void *out0, void *out1, size_t batch, void *stream);
 
 /* Real-side buffers (`in` of forward, `out` of backward and sqr-mag) hold F
  * values: float for ZLFFT_PREC_F32 plans, double for ZLFFT_PREC_F64 plans. */

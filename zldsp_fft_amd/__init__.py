@@ -59,7 +59,7 @@ EXPORTS = (
     "zlfft_ragged_create", "zlfft_ragged_create_config", "zlfft_ragged_exec", "zlfft_ragged_destroy",
     "zlfft_rfft_stft_exec",
     "zlfft_cfft_execute_batch", "zlfft_rfft_forward_batch", "zlfft_rfft_forward_sqr_mag_batch",
-    "zlfft_rfft_backward_batch",
+    "zlfft_rfft_backward_batch", "zlfft_cfft_exec_noalloc",
 )
 STFT_BINS, STFT_POWER, STFT_DB = 0, 1, 2
 FORWARD_SQR_MAG = 2
@@ -104,6 +104,7 @@ def load_library() -> ctypes.CDLL:
     L.zlfft_plan_get_order.argtypes = [vp]
     L.zlfft_plan_get_order.restype = sz
     L.zlfft_cfft_exec.argtypes = [vp, ci, ci, vp, vp, ci, vp, vp, sz, vp]
+    L.zlfft_cfft_exec_noalloc.argtypes = [vp, ci, ci, vp, vp, ci, vp, vp, sz, vp]
     L.zlfft_rfft_forward_exec.argtypes = [vp, vp, ci, vp, vp, sz, vp]
     L.zlfft_rfft_forward_sqr_mag_exec.argtypes = [vp, vp, vp, sz, vp]
     L.zlfft_rfft_backward_exec.argtypes = [vp, ci, vp, vp, vp, sz, vp]

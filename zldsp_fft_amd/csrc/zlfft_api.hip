@@ -71,6 +71,21 @@ void load_tune() {
 }
 }  // namespace
 
+// ---- scratch of the allocation-free entry point ---------------------------
+// The multi-pass intermediates of zlfft_cfft_exec_noalloc live in this array
+// (one copy per GPU, allocated with the code object and held until the
+// process ends).  Host code never uses its name as an address: the device
+// address comes from hipGetSymbolAddress, once per device, with that device
+// current, and is kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+}  // namespace
+
 int zl_tuned_ragged_logr(int order) {
     std::call_once(g_tune_once, load_tune);
     if (order < 0 || order > zl::kRaggedMaxOrderTune) return 4;
@@ -753,6 +768,23 @@ int run_host_chunks(zlfft_plan p, Buf in, Buf out, size_t batch, Body body) {
     return ZLFFT_OK;
 }
 
+// The device address of `scratch` on the current device (looked up once per
+// device).  The calling thread's current device must be the plan's.
+int scratch_ptr(zlfft_plan p, void** out) {
+    int dev = -1;
+    ZL_HIP(hipGetDevice(&dev));
+    if (dev != p->device) return fail(ZLFFT_ERR_INVALID_ARG, "the plan's device is not the current device");
+    if (dev < 0 || dev >= kMaxDevices) return fail(ZLFFT_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        ZL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return ZLFFT_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -986,6 +1018,32 @@ int zlfft_cfft_exec(zlfft_plan p, int direction, int in_layout, const void* in0,
         return fail(ZLFFT_ERR_NULL, "NULL data pointer");
     return launch(p, zl::C2C, in_layout, out_layout, in0, in1, out0, out1, batch, direction == ZLFFT_BACKWARD,
                   stream);
+}
+
+int zlfft_cfft_exec_noalloc(zlfft_plan p, int direction, int in_layout, const void* in0, const void* in1,
+                            int out_layout, void* out0, void* out1, size_t batch, void* stream) {
+    if (int rc = check_plan(p)) return rc;
+    if (p->kind != ZLFFT_KIND_CFFT) return fail(ZLFFT_ERR_INVALID_ARG, "plan is not a CFFT plan");
+    if (direction != ZLFFT_FORWARD && direction != ZLFFT_BACKWARD)
+        return fail(ZLFFT_ERR_INVALID_ARG, "direction must be ZLFFT_FORWARD or ZLFFT_BACKWARD");
+    if (!valid_layout(in_layout) || !valid_layout(out_layout)) return fail(ZLFFT_ERR_INVALID_ARG, "bad layout");
+    if (batch && (!in0 || !out0 || (in_layout == ZLFFT_SOA && !in1) || (out_layout == ZLFFT_SOA && !out1)))
+        return fail(ZLFFT_ERR_NULL, "NULL data pointer");
+    // one-kernel plans use no scratch: the same launch as zlfft_cfft_exec
+    if (!p->macro)
+        return launch(p, zl::C2C, in_layout, out_layout, in0, in1, out0, out1, batch, direction == ZLFFT_BACKWARD,
+                      stream);
+    if (batch == 0) return ZLFFT_OK;
+    const size_t need = macro_work_bytes(p, batch);
+    if (need > kScratchBytes)
+        return fail(ZLFFT_ERR_INVALID_ARG, "batch needs " + std::to_string(need) +
+                                               " bytes of intermediates; the allocation-free call holds at most " +
+                                               std::to_string(kScratchBytes));
+    void* work = nullptr;
+    if (int rc = scratch_ptr(p, &work)) return rc;
+    // every intermediate element is written before it is read: nothing to clear
+    return macro_cfft(p, direction == ZLFFT_BACKWARD, in_layout, in0, in1, out_layout, out0, out1, batch, stream,
+                      work, 3);
 }
 
 int zlfft_rfft_forward_exec(zlfft_plan p, const void* in, int out_layout, void* out0, void* out1, size_t batch,
