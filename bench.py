@@ -51,6 +51,7 @@ ROTATE = 5            # buffer sets per secondary line: > 256 MiB, so the MALL c
 
 
 def parse():
+    global WARM_S, MIN_TIMED
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -67,15 +68,22 @@ def parse():
                     help="comma-separated subset of the MFMA variant names")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
-    return ap.parse_args()
+    ap.add_argument("--warm-seconds", type=float, default=WARM_S,
+                    help="time-based warm-up per line (profiler runs use 0)")
+    ap.add_argument("--min-timed", type=int, default=MIN_TIMED,
+                    help="timed launches per secondary line and MFMA variant (profiler runs use a few)")
+    args = ap.parse_args()
+    WARM_S, MIN_TIMED = args.warm_seconds, args.min_timed
+    return args
 
 
 # ---------------------------------------------------------------- timing helpers
 
-def warm_up(torch, fns, seconds=WARM_S, min_calls=0):
+def warm_up(torch, fns, seconds=None, min_calls=0):
     """Back-to-back calls of fns (in turn) for about `seconds` of GPU work and
     at least min_calls calls.  Ends without a synchronisation, so the timed
     launches queued next follow the warm-up with no idle gap."""
+    seconds = WARM_S if seconds is None else min(seconds, WARM_S)
     probe = max(3, len(fns))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -125,9 +133,10 @@ def _line(nbytes, flops, count, ms, rel=None, kern_ms=None):
     return d
 
 
-def measure_line(torch, stream, fns, reps=MIN_TIMED):
+def measure_line(torch, stream, fns, reps=None):
     """Warm-up by time, then (wall ms per call of back-to-back launches,
     mean of per-launch event ms)."""
+    reps = MIN_TIMED if reps is None else min(reps, MIN_TIMED)
     warm_up(torch, fns)
     wall = time_window(torch, stream, fns, reps)
     warm_up(torch, fns, seconds=0.05)
@@ -244,10 +253,10 @@ def mfma_variants(x, y, stream, lib, args, plain_step) -> dict:
         steps[name] = step
     order = ["plain"] + [n for n, _ in variants]
     window = 10
-    rounds = max(10, -(-MIN_TIMED // window))
+    rounds = max(1, -(-MIN_TIMED // window))
     for name in order:
         warm_up(torch, [steps[name]], seconds=0.1)
-    warm_up(torch, [steps[n] for n in order], seconds=WARM_S)
+    warm_up(torch, [steps[n] for n in order])
     times = {n: [] for n in order}
     for _ in range(rounds):
         for name in order:
@@ -503,6 +512,14 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     sections = {"configs": config_lines, "ragged": ragged_section, "large": large_lines, "analyzer": analyzer_lines}
+    if args.copy_probe:
+        # known-size copy: reads and writes 512 MiB, for scaling HBM counters
+        a = torch.empty(1 << 27, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if args.only in sections:
         print(json.dumps({"section": args.only, args.only: sections[args.only](args)}), flush=True)
         return
@@ -518,18 +535,13 @@ def main():
     def step():
         _call(lib, lib.zlfft_cfft_exec, h, zf.FORWARD, zf.AOS, xp, None, zf.AOS, yp, None, BATCH, sp)
 
-    if args.copy_probe:
-        # known-size copy: reads and writes BATCH * N * 8 bytes, for scaling HBM counters
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
     if args.only == "mfma":
         print(json.dumps({"section": "mfma", "mfma": mfma_variants(x, y, stream, lib, args, step)}), flush=True)
         return
 
     # warm-up: --warmup steps, extended by time so the clocks settle; no
     # synchronisation before the timed region
-    warm_up(torch, [step], seconds=WARM_S, min_calls=args.warmup)
+    warm_up(torch, [step], min_calls=args.warmup)
     # timed region: exactly --steps steps, each launch bracketed by events on the launch stream
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     t_start = torch.cuda.Event(enable_timing=True)

@@ -979,6 +979,9 @@ fft_stream_kernel(KArgs a) {
 
 // ---------------------------------------------------------------- large N
 
+#ifndef ZL_MACRO_NT_MIN_ROW
+#define ZL_MACRO_NT_MIN_ROW 128
+#endif
 #ifndef ZL_MACRO_NT
 #define ZL_MACRO_NT 1
 #endif
@@ -1038,7 +1041,14 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     // next pass reads back from the cache (measured: the 4 Mi-point SoA
     // transforms of orders 16-22 gain 5-25%).
     constexpr bool nt_in = FIRST && NT && ZL_MACRO_NT;
-    constexpr bool nt_out = LAST && NT && ZL_MACRO_NT;
+    // A row of the last pass's output is C consecutive elements; below a
+    // whole 128-byte line (SoA at 16 columns: 64 bytes per array) the
+    // non-temporal stores reach memory as partial lines: measured 1.27x the
+    // output bytes in WRITE_SIZE for the radix-1024 SoA pass.  Such rows go
+    // through the cache, which merges the halves of a line that neighbouring
+    // column blocks (renumbered onto one XCD by xcd_block) write.
+    constexpr int row_bytes = C * ((LAST && LOUT == SOA) ? (int)sizeof(CX) / 2 : (int)sizeof(CX));
+    constexpr bool nt_out = LAST && NT && ZL_MACRO_NT && row_bytes >= ZL_MACRO_NT_MIN_ROW;
     CX v[R];
     if constexpr (nt_in) {
 #pragma unroll
