@@ -43,8 +43,56 @@ def _run(cmd: list[str]) -> tuple[list[str], int, str]:
     return cmd, p.returncode, err
 
 
+# The host layer (C-ABI, plans, host pipeline, ragged planning, tuning-file
+# parser) again with AddressSanitizer and UndefinedBehaviorSanitizer on the
+# host side only (each -fsanitize after -Xarch_host: the device code is the
+# normal build's), linked with the normal kernel objects into
+# _build/asan/libzlfft.so, and a CPU driver that walks its error paths
+# (tests/test_sanitize_cpu.py).
+ASAN_DIR = os.path.join(OBJ, "asan")
+ASAN_LIB = os.path.join(ASAN_DIR, "libzlfft.so")
+ASAN_UNITS = ("zlfft_api.hip", "zlfft_ragged_api.hip")
+SAN_FLAGS = ["-O1", "-g", "-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer"]
+DRIVER_SRC = os.path.join(ROOT, "tests", "sanitize", "host_abi_driver.cpp")
+DRIVER_BIN = os.path.join(ASAN_DIR, "host_abi_driver")
+
+
+def _clang_rt_dir() -> str:
+    base = "/opt/rocm/llvm/lib/clang"
+    for v in sorted(os.listdir(base)) if os.path.isdir(base) else []:
+        d = os.path.join(base, v, "lib", "linux")
+        if os.path.exists(os.path.join(d, "libclang_rt.asan-x86_64.so")):
+            return d
+    raise RuntimeError("the clang AddressSanitizer runtime is not installed")
+
+
+def link_sanitized() -> str:
+    """Links the sanitized host layer and the driver (objects compiled by build())."""
+    objs = [os.path.join(OBJ, f) for f in sorted(os.listdir(OBJ))
+            if f.endswith(".o") and f not in ("api.o", "ragged_api.o")]
+    objs += [os.path.join(ASAN_DIR, u.replace("zlfft_", "").replace(".hip", ".o")) for u in ASAN_UNITS]
+    vmap = os.path.join(OBJ, "exports.map")
+    _, rc, err = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address",
+                       "-Xarch_host", "-fsanitize=undefined", "-Xarch_host", "-shared-libasan",
+                       f"-Wl,--version-script={vmap}", "-o", ASAN_LIB, *objs])
+    if rc:
+        print(err, file=sys.stderr)
+        raise RuntimeError("link of the sanitized libzlfft.so failed")
+    clangxx = "/opt/rocm/llvm/bin/clang++"
+    rt = _clang_rt_dir()
+    _, rc, err = _run([clangxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-shared-libasan",
+                       "-fno-omit-frame-pointer", "-I", INCLUDE, DRIVER_SRC, "-o", DRIVER_BIN, "-L", ASAN_DIR, "-lzlfft",
+                       f"-Wl,-rpath,{ASAN_DIR}", f"-Wl,-rpath,{rt}"])
+    if rc:
+        print(err, file=sys.stderr)
+        raise RuntimeError("sanitizer driver build failed")
+    return DRIVER_BIN
+
+
 def build(jobs: int | None = None, verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(ASAN_DIR, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
     units = []
     for n in range(MAX_LDS_ORDER + 1):
@@ -56,6 +104,9 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     units.append((os.path.join(CSRC, "zlfft_ragged_api.hip"), os.path.join(OBJ, "ragged_api.o"), []))
     units.append((os.path.join(CSRC, "zlfft_api.hip"), os.path.join(OBJ, "api.o"), []))
     cmds = [[HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
+    for u in ASAN_UNITS:
+        cmds.append([HIPCC, *HIP_FLAGS, *SAN_FLAGS, "-c", os.path.join(CSRC, u),
+                     "-o", os.path.join(ASAN_DIR, u.replace("zlfft_", "").replace(".hip", ".o"))])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         for cmd, rc, err in ex.map(_run, cmds):
             if verbose or rc:
@@ -77,6 +128,7 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         raise RuntimeError("link of libzlfft.so failed")
     os.replace(tmp, LIB)
     build_dropin_example()
+    link_sanitized()
     return LIB
 
 

@@ -31,6 +31,21 @@ int fail(int code, const std::string& msg) {
 // shared with the other C-ABI translation units
 int zl_fail(int code, const std::string& msg) { return fail(code, msg); }
 
+// Fault injection, for the tests of the error paths: ZLFFT_INJECT, read when a
+// plan or ragged batch is created.  "alloc": that creation's device
+// allocations fail; "work_alloc": the plan is created, its later allocations
+// (multi-pass workspace, host-pipeline staging) fail; "launch": its kernel
+// launches fail.  Each failing call returns the code a real failure would
+// (ZLFFT_ERR_ALLOC, ZLFFT_ERR_HIP) and no kernel is launched in a failing state.
+int zl_read_inject() {
+    const char* e = getenv("ZLFFT_INJECT");
+    if (!e) return zl::kInjectNone;
+    if (!strcmp(e, "alloc")) return zl::kInjectAlloc;
+    if (!strcmp(e, "work_alloc")) return zl::kInjectWorkAlloc;
+    if (!strcmp(e, "launch")) return zl::kInjectLaunch;
+    return zl::kInjectNone;
+}
+
 // ---- tuning table -------------------------------------------------------
 // Text file of "<key> <order> <value>" lines written by
 // zldsp_fft_amd/autotune.py: $ZLFFT_TUNE_FILE, else zlfft_tune.txt next to
@@ -239,6 +254,7 @@ __global__ void gen_rfft_twiddles(CX* out, long long nreal, int count) {
 }  // namespace
 
 struct zlfft_plan_s {
+    int inject = zl::kInjectNone;  // fault injection (zl_read_inject)
     int kind = ZLFFT_KIND_CFFT;
     int precision = ZLFFT_PREC_F32;
     int order = 0;   // user order
@@ -288,6 +304,25 @@ struct zlfft_plan_s {
 
 namespace {
 
+// device allocation of a plan; fails as out of memory under the fault
+// injection `when` (kInjectAlloc at creation, kInjectWorkAlloc later)
+template <class T>
+hipError_t plan_malloc(zlfft_plan p, T** ptr, size_t bytes, int when) {
+    if (p->inject == when) return hipErrorOutOfMemory;
+    return hipMalloc(reinterpret_cast<void**>(ptr), bytes);
+}
+
+// kernel launch of a plan's call (fails under kInjectLaunch)
+hipError_t plan_launch(zlfft_plan p, const void* fn, dim3 grid, dim3 block, void** params, hipStream_t stream) {
+    if (p->inject == zl::kInjectLaunch) return hipErrorLaunchFailure;
+    return hipLaunchKernel(fn, grid, block, params, 0, stream);
+}
+
+// the same check before a launch written with the triple-chevron syntax
+hipError_t plan_launch_check(zlfft_plan p) {
+    return p->inject == zl::kInjectLaunch ? hipErrorLaunchFailure : hipSuccess;
+}
+
 int check_plan(zlfft_plan p) {
     if (!p) return fail(ZLFFT_ERR_NULL, "plan is NULL");
     return ZLFFT_OK;
@@ -317,7 +352,7 @@ int ensure_work(zlfft_plan p, size_t bytes) {
         p->d_work = nullptr;
         p->work_bytes = 0;
     }
-    hipError_t e = hipMalloc(&p->d_work, bytes);
+    hipError_t e = plan_malloc(p, &p->d_work, bytes, zl::kInjectWorkAlloc);
     if (e != hipSuccess) return fail(ZLFFT_ERR_ALLOC, std::string("workspace hipMalloc: ") + hipGetErrorString(e));
     p->work_bytes = bytes;
     return ZLFFT_OK;
@@ -357,8 +392,8 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
         const size_t grid = batch * (size_t)(n >> p->logl[q]) / (size_t)e.cols;
         if (grid > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
         void* params[] = {&a};
-        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)grid), dim3(e.wg), params, 0,
-                               reinterpret_cast<hipStream_t>(stream)));
+        ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.fn), dim3((unsigned)grid), dim3(e.wg), params,
+                           reinterpret_cast<hipStream_t>(stream)));
         s0 = d0;
         s1 = d1;
     }
@@ -381,6 +416,7 @@ int rfft_post(zlfft_plan p, int kind, int lout, const void* z, void* out0, void*
     const long long threads = (long long)batch * (long long)(n / 2 + 1);
     const unsigned grid = (unsigned)((threads + 255) / 256);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ZL_HIP(plan_launch_check(p));
     const CX* zz = static_cast<const CX*>(z);
     const CX* rtw = static_cast<const CX*>(p->d_rtw);
     if (kind == zl::R2C_SQR)
@@ -400,6 +436,7 @@ int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, s
     const long long threads = (long long)batch * (long long)(n / 2 + 1);
     const unsigned grid = (unsigned)((threads + 255) / 256);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ZL_HIP(plan_launch_check(p));
     CX* zz = static_cast<CX*>(z);
     const CX* rtw = static_cast<const CX*>(p->d_rtw);
     if (lin == zl::AOS)
@@ -479,7 +516,7 @@ int launch_mfma(zlfft_plan p, int lin, int lout, const void* in0, void* out0, si
     a.batch = (long long)batch;
     void* params[] = {&a};
     const unsigned grid = (unsigned)(streaming ? resident : (long long)batch);
-    ZL_HIP(hipLaunchKernel(fn, dim3(grid), dim3(256), params, 0, reinterpret_cast<hipStream_t>(stream)));
+    ZL_HIP(plan_launch(p, fn, dim3(grid), dim3(256), params, reinterpret_cast<hipStream_t>(stream)));
     return ZLFFT_OK;
 }
 
@@ -527,11 +564,11 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
         a.tw = p->d_img;
         a.sink = p->d_sink;
         a.cached_out = rfft_cached_out(batch * (((size_t)1 << p->corder) + 1) * 8);
-        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident), dim3(e.s_wg),
-                               params, 0, reinterpret_cast<hipStream_t>(stream)));
+        ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident),
+                           dim3(e.s_wg), params, reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;
     }
-    ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params, 0,
+    ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params,
                            reinterpret_cast<hipStream_t>(stream)));
     return ZLFFT_OK;
 }
@@ -626,7 +663,8 @@ int ensure_slot(zlfft_plan p, int i, size_t bytes) {
     if (sl.dev) (void)hipFree(sl.dev);
     sl.pin = sl.dev = nullptr;
     sl.bytes = 0;
-    if (hipHostMalloc(&sl.pin, bytes, hipHostMallocDefault) != hipSuccess || hipMalloc(&sl.dev, bytes) != hipSuccess)
+    if (p->inject == zl::kInjectWorkAlloc || hipHostMalloc(&sl.pin, bytes, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&sl.dev, bytes) != hipSuccess)
         return fail(ZLFFT_ERR_ALLOC, "host pipeline staging allocation failed");
     sl.bytes = bytes;
     return ZLFFT_OK;
@@ -813,6 +851,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
 
     zlfft_plan p = new (std::nothrow) zlfft_plan_s();
     if (!p) return fail(ZLFFT_ERR_ALLOC, "plan allocation failed");
+    p->inject = zl_read_inject();
     p->kind = kind;
     p->precision = precision;
     p->order = (int)order;
@@ -864,7 +903,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
             p->logw[q] = lw;
             const zl::MacroEntry& me = (f64 ? g_macro64 : g_macro)[mpl.logl[q]][0][0];
             if (me.tw_total > 0) {
-                e = hipMalloc(&p->d_itw[q], csz * (size_t)me.tw_total);
+                e = plan_malloc(p, &p->d_itw[q], csz * (size_t)me.tw_total, zl::kInjectAlloc);
                 if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
                 size_t off = 0;
                 for (int r = 1; r < me.np; ++r) {
@@ -878,8 +917,8 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
                 const int h = (log_lw + 1) / 2;
                 p->logh[q] = h;
                 const int na = 1 << (log_lw - h), nb = 1 << h;
-                e = hipMalloc(&p->d_twa[q], csz * (size_t)na);
-                if (e == hipSuccess) e = hipMalloc(&p->d_twb[q], csz * (size_t)nb);
+                e = plan_malloc(p, &p->d_twa[q], csz * (size_t)na, zl::kInjectAlloc);
+                if (e == hipSuccess) e = plan_malloc(p, &p->d_twb[q], csz * (size_t)nb, zl::kInjectAlloc);
                 if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
                 gen_macro(p->d_twa[q], log_lw, h, na);
                 gen_macro(p->d_twb[q], log_lw, 0, nb);
@@ -907,14 +946,14 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         }
         const zl::KernelEntry& ke = (*p->tab)[zl::C2C][0][0];
         if (ke.stream_fn[0] || (*p->tab)[zl::R2C][0][0].stream_fn[0]) {
-            e = hipMalloc(&p->d_sink, (size_t)zl::kSinkBytes);
+            e = plan_malloc(p, &p->d_sink, (size_t)zl::kSinkBytes, zl::kInjectAlloc);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "sink hipMalloc failed"));
         }
         // the LDS twiddle image of the plan's kind: C2C for CFFT plans, the
         // real kinds' (one geometry and image for all of them) for RFFT plans
         const zl::KernelEntry& ki = (*p->tab)[own_kind][0][0];
         if (ki.img_total > 0) {
-            e = hipMalloc(&p->d_img, csz * (size_t)ki.img_total);
+            e = plan_malloc(p, &p->d_img, csz * (size_t)ki.img_total, zl::kInjectAlloc);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
             size_t off = 0;
             for (int q = 1; q < ki.s_np; ++q) {
@@ -933,7 +972,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
     // pass twiddles of the plan's kind (the LDS kernels')
     const zl::KernelEntry& ent = p->macro ? zl::KernelEntry{} : (*p->tab)[own_kind][0][0];
     if (ent.tw_total > 0) {
-        e = hipMalloc(&p->d_tw, csz * (size_t)ent.tw_total);
+        e = plan_malloc(p, &p->d_tw, csz * (size_t)ent.tw_total, zl::kInjectAlloc);
         if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
         size_t off = 0;
         for (int q = 1; q < ent.np; ++q) {
@@ -942,7 +981,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
             off += (size_t)cnt;
         }
         if (mfma) {
-            e = hipMalloc(&p->d_twm, sizeof(float2) * (size_t)zl::kMfmaTwTotal);
+            e = plan_malloc(p, &p->d_twm, sizeof(float2) * (size_t)zl::kMfmaTwTotal, zl::kInjectAlloc);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "twiddle hipMalloc failed"));
             const bool f16 = precision == ZLFFT_PREC_F16_MFMA || precision == ZLFFT_PREC_F16_MFMA_SPLIT;
             gen_mfma_twiddles<<<(zl::kMfmaTwTotal + 255) / 256, 256, 0, p->stream>>>(p->d_twm, f16 ? 0.0625f : 1.0f);
@@ -954,7 +993,7 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
         const long long nreal = 1ll << order;
         p->n_rtw = (int)(nreal / 4);
         if (p->n_rtw > 0) {
-            e = hipMalloc(&p->d_rtw, csz * (size_t)p->n_rtw);
+            e = plan_malloc(p, &p->d_rtw, csz * (size_t)p->n_rtw, zl::kInjectAlloc);
             if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "rfft twiddle hipMalloc failed"));
             const unsigned g = (unsigned)((p->n_rtw + 255) / 256);
             if (f64)
@@ -1187,12 +1226,12 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
             a.tw = p->d_img;
             a.sink = p->d_sink;
             a.cached_out = rfft_cached_out(nframes * (((size_t)1 << p->corder) + 1) * 8);
-            ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.stream_fn[0]), dim3((unsigned)resident),
-                                   dim3(e.s_wg), params, 0, reinterpret_cast<hipStream_t>(stream)));
+            ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.stream_fn[0]), dim3((unsigned)resident),
+                               dim3(e.s_wg), params, reinterpret_cast<hipStream_t>(stream)));
             return ZLFFT_OK;
         }
-        ZL_HIP(hipLaunchKernel(reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params, 0,
-                               reinterpret_cast<hipStream_t>(stream)));
+        ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.fn), dim3((unsigned)groups), dim3(e.wg), params,
+                           reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;
     }
     // multi-pass sizes: the windowed frames go to the workspace (after the
@@ -1205,6 +1244,7 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
     const long long total = (long long)(nframes * n);
     const unsigned grid = (unsigned)((total + 255) / 256);
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    ZL_HIP(plan_launch_check(p));
     if (p->f64)
         zl::frame_kernel<double><<<grid, 256, 0, s>>>(static_cast<const double*>(signal),
                                                       static_cast<const double*>(window), static_cast<double*>(frames),

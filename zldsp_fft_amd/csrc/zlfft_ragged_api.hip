@@ -65,6 +65,7 @@ long long item_bytes(int corder, int key) {
 }  // namespace
 
 struct zlfft_ragged_s {
+    int inject = zl::kInjectNone;  // fault injection (zl_read_inject)
     zl::RDesc* d_desc = nullptr;
     zl::RItem* d_items = nullptr;
     int* d_runs = nullptr;
@@ -217,6 +218,7 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
 
     zlfft_ragged r = new (std::nothrow) zlfft_ragged_s();
     if (!r) return zl_fail(ZLFFT_ERR_ALLOC, "allocation failed");
+    r->inject = zl_read_inject();
     auto cleanup = [&](int rc) {
         zlfft_ragged_destroy(r);
         return rc;
@@ -224,7 +226,7 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
     r->nitems = (int)items.size();
     e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return cleanup(hipf(e, "hipStreamCreateWithFlags"));
-    if (hipMalloc(&r->d_desc, sizeof(RDesc) * hd.size()) != hipSuccess ||
+    if (r->inject == kInjectAlloc || hipMalloc(&r->d_desc, sizeof(RDesc) * hd.size()) != hipSuccess ||
         hipMalloc(&r->d_items, sizeof(RItem) * std::max<size_t>(items.size(), 1)) != hipSuccess ||
         hipMalloc(&r->d_runs, sizeof(int) * runs.size()) != hipSuccess ||
         hipMalloc(&r->d_mt, sizeof(float2) * kMtTotal) != hipSuccess ||
@@ -271,7 +273,8 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
 int zlfft_ragged_exec(zlfft_ragged r, void* stream) {
     if (!r) return zl_fail(ZLFFT_ERR_NULL, "ragged handle is NULL");
     if (r->nitems == 0) return ZLFFT_OK;
-    RG_HIP(hipGraphLaunch(r->exec, reinterpret_cast<hipStream_t>(stream)));
+    RG_HIP(r->inject == zl::kInjectLaunch ? hipErrorLaunchFailure
+                                          : hipGraphLaunch(r->exec, reinterpret_cast<hipStream_t>(stream)));
     return ZLFFT_OK;
 }
 

@@ -186,7 +186,13 @@ MacroEntry make_macro_entry() {
     return e;
 }
 
+// fault-injection modes of ZLFFT_INJECT (zl_read_inject, the error-path tests)
+constexpr int kInjectNone = 0, kInjectAlloc = 1, kInjectWorkAlloc = 2, kInjectLaunch = 3;
+
 }  // namespace zl
+
+// ZLFFT_INJECT as a kInject* mode (zlfft_api.hip)
+int zl_read_inject();
 
 // MFMA split-precision 4096-point kernel for a precision code (one transform
 // per workgroup, or the persistent streaming form), or null
