@@ -69,8 +69,7 @@ def _clang_rt_dir() -> str:
 
 def link_sanitized() -> str:
     """Links the sanitized host layer and the driver (objects compiled by build())."""
-    objs = [os.path.join(OBJ, f) for f in sorted(os.listdir(OBJ))
-            if f.endswith(".o") and f not in ("api.o", "ragged_api.o")]
+    objs = [obj for _, obj, _ in _units() if os.path.basename(obj) not in ("api.o", "ragged_api.o")]
     objs += [os.path.join(ASAN_DIR, u.replace("zlfft_", "").replace(".hip", ".o")) for u in ASAN_UNITS]
     vmap = os.path.join(OBJ, "exports.map")
     _, rc, err = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address",
@@ -90,10 +89,8 @@ def link_sanitized() -> str:
     return DRIVER_BIN
 
 
-def build(jobs: int | None = None, verbose: bool = False) -> str:
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(ASAN_DIR, exist_ok=True)
-    jobs = jobs or min(8, os.cpu_count() or 4)
+def _units() -> list[tuple[str, str, list[str]]]:
+    """(source, object, extra flags) of every compilation unit of libzlfft.so."""
     units = []
     for n in range(MAX_LDS_ORDER + 1):
         units.append((os.path.join(CSRC, "zlfft_order.hip"), os.path.join(OBJ, f"order{n}.o"), [f"-DZL_LOGN={n}"]))
@@ -103,6 +100,14 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
     units.append((os.path.join(CSRC, "zlfft_ragged_k.hip"), os.path.join(OBJ, "ragged_k.o"), []))
     units.append((os.path.join(CSRC, "zlfft_ragged_api.hip"), os.path.join(OBJ, "ragged_api.o"), []))
     units.append((os.path.join(CSRC, "zlfft_api.hip"), os.path.join(OBJ, "api.o"), []))
+    return units
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    units = _units()
     cmds = [[HIPCC, *HIP_FLAGS, *extra, "-c", src, "-o", obj] for src, obj, extra in units]
     for u in ASAN_UNITS:
         cmds.append([HIPCC, *HIP_FLAGS, *SAN_FLAGS, "-c", os.path.join(CSRC, u),
