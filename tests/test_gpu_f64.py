@@ -146,6 +146,37 @@ def test_rfft64(order):
         assert signals.rel_l2(fetch(o), refb).max() <= TOL, (order, lin)
 
 
+@pytest.mark.parametrize("order", [5, 8, 10, 11, 12, 13])
+def test_rfft64_streaming_batches(order):
+    """RFFT<double> batches larger than the resident workgroups: the persistent
+    streaming kernel (real orders up to 12) and the one-launch kernel above,
+    AoS / SoA bins and squared magnitudes on sampled frames against the FP64
+    oracle, the part-filled last group included; DC and Nyquist exactly real."""
+    n = 1 << order
+    nb = n // 2 + 1
+    batch = max(4096, (1 << 22) // n) + 3
+    g = torch.Generator(device="cuda").manual_seed(600 + order)
+    xd = torch.rand((batch, n), dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    plan = zf.RFFT(order, zf.PREC_F64)
+    idx = np.unique(np.linspace(0, batch - 1, 10).astype(int))
+    x = xd[idx].cpu().numpy()
+    ref = np.stack([oracle.rfft_forward(r, "f64") for r in x])
+    for lout in (zf.AOS, zf.SOA):
+        y = make_out((batch, nb), lout)
+        plan.forward(xd, y)
+        torch.cuda.synchronize()
+        got = fetch(tuple(t[idx] for t in y) if isinstance(y, tuple) else y[idx])
+        e = signals.rel_l2(got, ref).max()
+        print(f"f64 rfft stream order={order} batch={batch} lout={lout} relL2={e:.2e}", flush=True)
+        assert e <= TOL, (order, lout)
+        assert np.all(got[:, 0].imag == 0.0) and np.all(got[:, -1].imag == 0.0)
+    p = torch.full((batch, nb), np.nan, dtype=torch.float64, device="cuda")
+    plan.forward_sqr_mag(xd, p)
+    torch.cuda.synchronize()
+    assert signals.rel_l2(p[idx].cpu().numpy(), np.abs(ref) ** 2).max() <= TOL, order
+    assert not torch.isnan(p).any()
+
+
 def test_f64_host_pointers_and_dtype_checks():
     """Synchronous host-pointer calls (numpy complex128 / float64) equal the
     device batch bit for bit; FP32 buffers are refused by an FP64 plan."""

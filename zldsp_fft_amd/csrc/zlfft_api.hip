@@ -563,7 +563,7 @@ int launch(zlfft_plan p, int kind, int lin, int lout, const void* in0, const voi
         // persistent streaming kernel: one workgroup per resident slot
         a.tw = p->d_img;
         a.sink = p->d_sink;
-        a.cached_out = rfft_cached_out(batch * (((size_t)1 << p->corder) + 1) * 8);
+        a.cached_out = rfft_cached_out(batch * (((size_t)1 << p->corder) + 1) * 2 * p->ss);
         ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.stream_fn[backward ? 1 : 0]), dim3((unsigned)resident),
                            dim3(e.s_wg), params, reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;
@@ -1225,7 +1225,7 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
             // persistent streaming kernel, as in launch()
             a.tw = p->d_img;
             a.sink = p->d_sink;
-            a.cached_out = rfft_cached_out(nframes * (((size_t)1 << p->corder) + 1) * 8);
+            a.cached_out = rfft_cached_out(nframes * (((size_t)1 << p->corder) + 1) * 2 * p->ss);
             ZL_HIP(plan_launch(p, reinterpret_cast<const void*>(e.stream_fn[0]), dim3((unsigned)resident),
                                dim3(e.s_wg), params, reinterpret_cast<hipStream_t>(stream)));
             return ZLFFT_OK;

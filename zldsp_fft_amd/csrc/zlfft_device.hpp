@@ -755,7 +755,9 @@ __global__ void __launch_bounds__(G::WG) __attribute__((amdgpu_waves_per_eu(ZL_S
 fft_stream_kernel(KArgs a) {
     static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR, "streaming kernel handles C2C and R2C");
     static_assert(!FRAMED || KIND != C2C, "framed input is real");
-    static_assert(sizeof(CX) == 8 || KIND == C2C, "FP64 streaming kernel: C2C only");
+    static_assert(sizeof(CX) == 8 || !FRAMED, "FP64 streaming kernel: not framed");
+    using XT = CxT<CX>;
+    using S = typename XT::S;
     constexpr int DMAX = cfg_stream_direct_max<CX>(G::logn, KIND);
     using TI = TwImage<G, DMAX>;
     __shared__ CX lds[G::TPW * G::LDSN];
@@ -902,14 +904,14 @@ fft_stream_kernel(KArgs a) {
             // the framed kernels); cached_tag: cached or non-temporal stores
             auto epilogue = [&](auto pw, auto cached_tag) {
                 constexpr bool cached = decltype(cached_tag)::value;
-                auto st2 = [&](float2* q, float2 x) {
+                auto st2 = [&](CX* q, CX x) {
                     if constexpr (cached) *q = x; else st_nt(q, x);
                 };
-                auto st1 = [&](float* q, float x) {
+                auto st1 = [&](S* q, S x) {
                     if constexpr (cached) *q = x; else st_nt(q, x);
                 };
-                const float2 z0 = slds[0];
-                const float dc = z0.x + z0.y, ny = z0.x - z0.y;
+                const CX z0 = slds[0];
+                const S dc = z0.x + z0.y, ny = z0.x - z0.y;
                 // bins k and M - k: put(k, X[k] or, for k = M / 2, X[M - k], X[M - k])
                 auto bins = [&](auto put) {
 #pragma unroll
@@ -920,45 +922,45 @@ fft_stream_kernel(KArgs a) {
                         int tt = t;
                         asm volatile("" : "+v"(tt));
                         const int k = 1 + tt + G::T * i;
-                        const float2 z1 = slds[G::pad(k)];
-                        const float2 z2 = slds[G::pad(M - k)];
-                        const float2 w = lrtw[k - 1];
-                        const float si = z1.y + z2.y, dr = z1.x - z2.x;
-                        const float2 xm = make_float2(__builtin_fmaf(dr, w.y, __builtin_fmaf(-si, w.x, z2.x)),
-                                                      __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z2.y)));
-                        const float2 xk = make_float2(__builtin_fmaf(-dr, w.y, __builtin_fmaf(si, w.x, z1.x)),
-                                                      __builtin_fmaf(-dr, w.x, __builtin_fmaf(-si, w.y, z1.y)));
+                        const CX z1 = slds[G::pad(k)];
+                        const CX z2 = slds[G::pad(M - k)];
+                        const CX w = lrtw[k - 1];
+                        const S si = z1.y + z2.y, dr = z1.x - z2.x;
+                        const CX xm = XT::mk(XT::fma(dr, w.y, XT::fma(-si, w.x, z2.x)),
+                                             XT::fma(-dr, w.x, XT::fma(-si, w.y, z2.y)));
+                        const CX xk = XT::mk(XT::fma(-dr, w.y, XT::fma(si, w.x, z1.x)),
+                                             XT::fma(-dr, w.x, XT::fma(-si, w.y, z1.y)));
                         put(k, k != M - k ? xk : xm, xm);
                     }
                 };
                 if constexpr (KIND == R2C && LOUT == AOS) {
-                    float2* const o = static_cast<float2*>(o0) + fb;
-                    st2(o, make_float2(dc, 0.f));
-                    st2(o + M, make_float2(ny, 0.f));
-                    bins([&](int k, float2 xk, float2 xm) {
+                    CX* const o = static_cast<CX*>(o0) + fb;
+                    st2(o, XT::mk(dc, S(0)));
+                    st2(o + M, XT::mk(ny, S(0)));
+                    bins([&](int k, CX xk, CX xm) {
                         st2(o + k, xk);
                         st2(o + (M - k), xm);
                     });
                 } else if constexpr (KIND == R2C) {
-                    float* const q0 = static_cast<float*>(o0) + fb;
-                    float* const q1 = static_cast<float*>(o1) + fb;
+                    S* const q0 = static_cast<S*>(o0) + fb;
+                    S* const q1 = static_cast<S*>(o1) + fb;
                     st1(q0, dc);
-                    st1(q1, 0.f);
+                    st1(q1, S(0));
                     st1(q0 + M, ny);
-                    st1(q1 + M, 0.f);
-                    bins([&](int k, float2 xk, float2 xm) {
+                    st1(q1 + M, S(0));
+                    bins([&](int k, CX xk, CX xm) {
                         st1(q0 + k, xk.x);
                         st1(q1 + k, xk.y);
                         st1(q0 + (M - k), xm.x);
                         st1(q1 + (M - k), xm.y);
                     });
                 } else {
-                    float* const q = static_cast<float*>(o0) + fb;
+                    S* const q = static_cast<S*>(o0) + fb;
                     st1(q, pw(dc * dc));
                     st1(q + M, pw(ny * ny));
-                    bins([&](int k, float2 xk, float2 xm) {
-                        st1(q + k, pw(__builtin_fmaf(xk.x, xk.x, xk.y * xk.y)));
-                        st1(q + (M - k), pw(__builtin_fmaf(xm.x, xm.x, xm.y * xm.y)));
+                    bins([&](int k, CX xk, CX xm) {
+                        st1(q + k, pw(XT::fma(xk.x, xk.x, xk.y * xk.y)));
+                        st1(q + (M - k), pw(XT::fma(xm.x, xm.x, xm.y * xm.y)));
                     });
                 }
             };
@@ -966,9 +968,9 @@ fft_stream_kernel(KArgs a) {
                 if (a.cached_out) epilogue(pw, std::true_type{}); else epilogue(pw, std::false_type{});
             };
             if (FRAMED && KIND == R2C_SQR && a.db)
-                with_pw([&](float q) { return power_out(q, 1, a.db_floor); });
+                with_pw([&](S q) { return power_out(q, 1, a.db_floor); });
             else
-                with_pw([](float q) { return q; });
+                with_pw([](S q) { return q; });
         }
     };
     if (g < ngroups) {

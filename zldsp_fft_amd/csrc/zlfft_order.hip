@@ -42,7 +42,7 @@ void ZL_CAT(zl_fill_order64_, ZL_LOGN)(zl::OrderTable& tab) {
     using namespace zl;
 #if ZL_LOGN <= 13
     using G = Geo<ZL_LOGN, cfg_logr(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
-    // the C2C streaming variant at 8 points per thread: at 16 its forward
+    // the streaming variants at 8 points per thread: at 16 the C2C forward
     // kernel took 256 VGPRs and scratch at N = 4096 (1 wave per SIMD); at 8
     // it takes 114-128, and two workgroups (2 x (69.6 KB tile + image)) fit
     // one CU's LDS (measured, 8192 x 4096: 0.198-0.200 ms against 0.208-0.210
@@ -52,9 +52,9 @@ void ZL_CAT(zl_fill_order64_, ZL_LOGN)(zl::OrderTable& tab) {
     tab[C2C][AOS][SOA] = make_lds_entry<G, C2C, AOS, SOA, double2, 0, GS>();
     tab[C2C][SOA][AOS] = make_lds_entry<G, C2C, SOA, AOS, double2, 0, GS>();
     tab[C2C][SOA][SOA] = make_lds_entry<G, C2C, SOA, SOA, double2, 0, GS>();
-    tab[R2C][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS>();
-    tab[R2C][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA>();
-    tab[R2C_SQR][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS>();
+    tab[R2C][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, double2, 0, GS>();
+    tab[R2C][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA, double2, 0, GS>();
+    tab[R2C_SQR][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS, double2, 0, GS>();
     tab[C2R][AOS][AOS] = make_lds_entry<G, C2R, AOS, AOS>();
     tab[C2R][SOA][AOS] = make_lds_entry<G, C2R, SOA, AOS>();
     tab[R2C_FRAMED][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, double2, 1>();

@@ -38,7 +38,8 @@ using OrderTable = KernelEntry[6][2][2];
 
 // The persistent streaming variant, for geometries whose direct I/O is
 // coalesced and whose LDS (transforms, twiddle image, RFFT twiddles) fits:
-// FP32, and FP64 C2C where two workgroups fit one CU.
+// FP32, and FP64 (C2C and the forward real kinds) where two workgroups fit
+// one CU.
 template <class G, int KIND, int LIN, int LOUT, int FRAMED = 0, class CX = float2>
 void add_stream_variant(KernelEntry& e) {
     constexpr int cs = (int)sizeof(CX);
@@ -47,7 +48,7 @@ void add_stream_variant(KernelEntry& e) {
     using TI = TwImage<G, cfg_stream_direct_max<CX>(G::logn, KIND)>;
     constexpr long long stream_lds = (long long)cs * (G::TPW * G::LDSN + TI::total() + (KIND == C2C ? 1 : G::N / 2) +
                                                       (FRAMED ? G::N : 0));
-    constexpr bool fits = cs == 8 ? stream_lds <= 160 * 1024 : KIND == C2C && !FRAMED && 2 * stream_lds <= 160 * 1024;
+    constexpr bool fits = cs == 8 ? stream_lds <= 160 * 1024 : !FRAMED && 2 * stream_lds <= 160 * 1024;
     if constexpr (KIND != C2R && direct && G::NP >= 2 && fits) {
         e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED, CX>;
         if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1, 0, CX>;
@@ -83,9 +84,9 @@ KernelEntry make_entry() {
     return e;
 }
 
-// Entry over complex type CX: FP64 (the LDS kernel, and for C2C the
-// streaming variant of geometry GS), and the framed STFT kernels of both
-// precisions (FP32 with the streaming variant)
+// Entry over complex type CX: FP64 (the LDS kernel, and for C2C and the
+// forward real kinds the streaming variant of geometry GS), and the framed
+// STFT kernels of both precisions (FP32 with the streaming variant)
 template <class G, int KIND, int LIN, int LOUT, class CX = double2, int FRAMED = 0, class GS = G>
 KernelEntry make_lds_entry() {
     static_assert((long long)sizeof(CX) * G::TPW * G::LDSN <= 160 * 1024, "geometry exceeds the LDS");
@@ -101,7 +102,7 @@ KernelEntry make_lds_entry() {
     }
     e.tw_total = G::tw_total();
     if constexpr (FRAMED && sizeof(CX) == 8) add_stream_variant<G, KIND, LIN, LOUT, FRAMED>(e);
-    if constexpr (!FRAMED && sizeof(CX) == 16 && KIND == C2C) add_stream_variant<GS, KIND, LIN, LOUT, 0, CX>(e);
+    if constexpr (!FRAMED && sizeof(CX) == 16 && KIND != C2R) add_stream_variant<GS, KIND, LIN, LOUT, 0, CX>(e);
     return e;
 }
 
