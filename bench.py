@@ -216,8 +216,10 @@ def cpu_baseline(seconds: float) -> dict:
         "sample": f"{sample} x 4096-point CFFT<float> forward, AoS, median of {nrun} runs on {t_all} threads; "
                   f"scalar C restatement of the reference ({path} plan, switch order {sw}), -O3 -march=x86-64-v3",
         "transforms_per_s": round(sample / t, 1),
+        "gflops_5nlog2n": round(5 * N * ORDER * sample / t / 1e9, 2),
         "t1": {"value": round(16 * N * 128 / t1 / 1e9, 3), "unit": "GB/s", "cores": 1,
-               "sample": f"128 transforms, median of {nrun1} runs", "transforms_per_s": round(128 / t1, 1)},
+               "sample": f"128 transforms, median of {nrun1} runs", "transforms_per_s": round(128 / t1, 1),
+               "gflops_5nlog2n": round(5 * N * ORDER * 128 / t1 / 1e9, 2)},
         "cpu_model": cpus["model"],
         "host_cpus": cpus,
     }
