@@ -447,10 +447,9 @@ int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, s
     return ZLFFT_OK;
 }
 
-// user_in: in0 / in1 is the caller's input (0: frames the STFT just wrote).
-// The workspace serves one stream at a time: a call on another stream than the previous call's first waits for
-// that call's work (not inside a stream capture, where the caller orders the
-// captured work).
+// The multi-pass transform of one call, on a workspace the caller has sized
+// (ensure_work) and ordered for `stream` (work_acquire).  user_in: in0 / in1
+// is the caller's input (0: frames the STFT just wrote).
 int launch_macro_body(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0,
                       void* out1, size_t batch, int backward, void* stream, int db, float db_floor, int user_in) {
     void* work = p->d_work;
@@ -470,7 +469,10 @@ int launch_macro_body(zlfft_plan p, int kind, int lin, int lout, const void* in0
     return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work, 2);
 }
 
-// orders `stream` after the previous workspace user (see launch_macro_body)
+// The workspace serves one stream at a time: a call on another stream than
+// the previous multi-pass call's waits, on the device, for that call's work
+// (work_ev, recorded by work_release).  Not inside a stream capture, where
+// the caller orders the captured work.
 int work_acquire(zlfft_plan p, void* stream) {
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (p->work_ev_set && p->work_stream != s && !capturing(stream)) ZL_HIP(hipStreamWaitEvent(s, p->work_ev, 0));
