@@ -750,7 +750,10 @@ __device__ __forceinline__ void run_pass_img(CX (&v)[G::R], CX* slds, const CX* 
 #ifndef ZL_STREAM_MIN_WAVES
 #define ZL_STREAM_MIN_WAVES 1
 #endif
-template <class G, int KIND, int LIN, int LOUT, int BWD, int FRAMED = 0, class CX = float2>
+// RTWG (R2C, R2C_SQR): the RFFT twiddles are read from global memory (the
+// L2 holds the table) instead of an LDS copy, for sizes whose LDS copy would
+// not fit beside the transform tile or would halve the resident workgroups.
+template <class G, int KIND, int LIN, int LOUT, int BWD, int FRAMED = 0, class CX = float2, int RTWG = 0>
 __global__ void __launch_bounds__(G::WG) __attribute__((amdgpu_waves_per_eu(ZL_STREAM_MIN_WAVES, 8)))
 fft_stream_kernel(KArgs a) {
     static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR, "streaming kernel handles C2C and R2C");
@@ -762,7 +765,9 @@ fft_stream_kernel(KArgs a) {
     using TI = TwImage<G, DMAX>;
     __shared__ CX lds[G::TPW * G::LDSN];
     __shared__ CX ltw[TI::total() > 0 ? TI::total() : 1];
-    __shared__ CX lrtw[KIND == C2C ? 1 : (G::N / 2 > 0 ? G::N / 2 : 1)];
+    static_assert(!RTWG || KIND != C2C, "RFFT twiddles: real kinds only");
+    __shared__ CX lrtw[KIND == C2C || RTWG ? 1 : (G::N / 2 > 0 ? G::N / 2 : 1)];
+    const CX* const grtw = static_cast<const CX*>(a.rtw);
     const int tid = threadIdx.x;
     const int slot = tid / G::T;
     const int t = tid - slot * G::T;
@@ -777,7 +782,7 @@ fft_stream_kernel(KArgs a) {
     // so the kernel's start waits for one memory round trip in all, not one
     // per row of each table (measured: the 4096-frame STFT 7% faster)
     constexpr int NI = (TI::total() + G::WG - 1) / G::WG;
-    constexpr int NR = KIND == C2C ? 0 : (M / 2 + G::WG - 1) / G::WG;
+    constexpr int NR = KIND == C2C || RTWG ? 0 : (M / 2 + G::WG - 1) / G::WG;
     constexpr int NW = FRAMED ? (M + G::WG - 1) / G::WG : 0;
     CX im[NI > 0 ? NI : 1], rr[NR > 0 ? NR : 1];
     float2 ww[NW > 0 ? NW : 1];
@@ -924,7 +929,7 @@ fft_stream_kernel(KArgs a) {
                         const int k = 1 + tt + G::T * i;
                         const CX z1 = slds[G::pad(k)];
                         const CX z2 = slds[G::pad(M - k)];
-                        const CX w = lrtw[k - 1];
+                        const CX w = RTWG ? grtw[k - 1] : lrtw[k - 1];
                         const S si = z1.y + z2.y, dr = z1.x - z2.x;
                         const CX xm = XT::mk(XT::fma(dr, w.y, XT::fma(-si, w.x, z2.x)),
                                              XT::fma(-dr, w.x, XT::fma(-si, w.y, z2.y)));

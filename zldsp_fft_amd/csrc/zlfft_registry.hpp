@@ -36,6 +36,10 @@ struct KernelEntry {
 // R2C_FRAMED / R2C_SQR_FRAMED); unused combinations stay null
 using OrderTable = KernelEntry[6][2][2];
 
+#ifndef ZL_RTW_GLOBAL_MIN
+#define ZL_RTW_GLOBAL_MIN 14
+#endif
+
 // The persistent streaming variant, for geometries whose direct I/O is
 // coalesced and whose LDS (transforms, twiddle image, RFFT twiddles) fits:
 // FP32, and FP64 (C2C and the forward real kinds) where two workgroups fit
@@ -46,11 +50,14 @@ void add_stream_variant(KernelEntry& e) {
     constexpr int elem = (KIND == C2C && (LIN == SOA || LOUT == SOA)) ? cs / 2 : cs;
     constexpr bool direct = G::T * elem >= 128 || G::TPW == 1;
     using TI = TwImage<G, cfg_stream_direct_max<CX>(G::logn, KIND)>;
-    constexpr long long stream_lds = (long long)cs * (G::TPW * G::LDSN + TI::total() + (KIND == C2C ? 1 : G::N / 2) +
-                                                      (FRAMED ? G::N : 0));
+    // FP32 forward real kinds of inner order >= ZL_RTW_GLOBAL_MIN read the
+    // RFFT twiddles from global memory (RTWG)
+    constexpr bool rtwg = cs == 8 && !FRAMED && KIND != C2C && G::logn >= ZL_RTW_GLOBAL_MIN;
+    constexpr long long stream_lds =
+        (long long)cs * (G::TPW * G::LDSN + TI::total() + (KIND == C2C || rtwg ? 1 : G::N / 2) + (FRAMED ? G::N : 0));
     constexpr bool fits = cs == 8 ? stream_lds <= 160 * 1024 : !FRAMED && 2 * stream_lds <= 160 * 1024;
     if constexpr (KIND != C2R && direct && G::NP >= 2 && fits) {
-        e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED, CX>;
+        e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED, CX, rtwg ? 1 : 0>;
         if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1, 0, CX>;
         e.s_wg = G::WG;
         e.s_tpw = G::TPW;
