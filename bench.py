@@ -63,9 +63,11 @@ def parse():
     ap.add_argument("--no-mfma", action="store_true", help="skip the MFMA split-precision variants")
     ap.add_argument("--no-configs", action="store_true", help="skip the analyzer and large-order lines")
     ap.add_argument("--no-ragged", action="store_true", help="skip the ragged-batch line")
-    ap.add_argument("--only", choices=("main", "mfma", "configs", "ragged", "large", "analyzer"), default=None,
+    ap.add_argument("--only", choices=("main", "mfma", "configs", "ragged", "large", "analyzer", "orders"),
+                    default=None,
                     help="run one section alone (for profiler runs): the headline batch, the MFMA variants, "
-                         "the secondary lines, the ragged batch, the large-order lines or the analyzer lines")
+                         "the secondary lines, the ragged batch, the large-order lines, the analyzer lines, or "
+                         "the plain batched path by size (orders: only this way, not in the default run)")
     ap.add_argument("--mfma-precisions", default=None,
                     help="comma-separated subset of the MFMA variant names")
     ap.add_argument("--copy-probe", action="store_true",
@@ -425,6 +427,53 @@ def large_lines(args) -> dict:
     return out
 
 
+def order_lines(args) -> dict:
+    """The plain batched path by size (not part of the default run): CFFT
+    orders 3-14 (AoS forward) and RFFT orders 4-15 (AoS bins and squared
+    magnitudes), 2^26 points per call, through the C-ABI over ROTATE buffer
+    sets; the persistent streaming kernel runs wherever the order has one."""
+    import torch
+    import zldsp_fft_amd as zf
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+    lib = zf.load_library()
+    out = {}
+    pts = 1 << 26
+    for order in range(3, 15):
+        n = 1 << order
+        batch = pts // n
+        plan = zf.CFFT(order)
+        h = plan._h
+        ins = [torch.rand((batch, n), dtype=torch.complex64, device=dev) for _ in range(ROTATE)]
+        outs = [torch.empty_like(t) for t in ins]
+        fns = [lambda i=i: _call(lib, lib.zlfft_cfft_exec, h, zf.FORWARD, zf.AOS, ins[i].data_ptr(), None, zf.AOS,
+                                 outs[i].data_ptr(), None, batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        out[f"cfft_o{order}_aos"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall, None, kern)
+        plan.close()
+        del ins, outs
+        rn, nb = 2 * n, n + 1  # RFFT of real order order + 1
+        plan = zf.RFFT(order + 1)
+        h = plan._h
+        ins = [torch.rand((batch, rn), device=dev) for _ in range(ROTATE)]
+        bins = [torch.empty((batch, nb), dtype=torch.complex64, device=dev) for _ in range(ROTATE)]
+        sq = [torch.empty((batch, nb), device=dev) for _ in range(ROTATE)]
+        fns = [lambda i=i: _call(lib, lib.zlfft_rfft_forward_exec, h, ins[i].data_ptr(), zf.AOS, bins[i].data_ptr(),
+                                 None, batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        flops = 2.5 * rn * (order + 1) * batch
+        out[f"rfft_o{order + 1}_bins"] = _line((4 * rn + 8 * nb) * batch, flops, batch, wall, None, kern)
+        fns = [lambda i=i: _call(lib, lib.zlfft_rfft_forward_sqr_mag_exec, h, ins[i].data_ptr(), sq[i].data_ptr(),
+                                 batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        out[f"rfft_o{order + 1}_sqr_mag"] = _line((4 * rn + 4 * nb) * batch, flops, batch, wall, None, kern)
+        plan.close()
+        del ins, bins, sq
+        torch.cuda.empty_cache()
+    return out
+
+
 def config_lines(args) -> dict:
     """Secondary lines: the analyzer batch and STFT, CFFT<double>
     (8192 x 4096 points), the headline batch from host memory (PCIe-inclusive,
@@ -515,7 +564,8 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    sections = {"configs": config_lines, "ragged": ragged_section, "large": large_lines, "analyzer": analyzer_lines}
+    sections = {"configs": config_lines, "ragged": ragged_section, "large": large_lines, "analyzer": analyzer_lines,
+                "orders": order_lines}
     if args.copy_probe:
         # known-size copy: reads and writes 512 MiB, for scaling HBM counters
         a = torch.empty(1 << 27, device="cuda")
