@@ -44,6 +44,7 @@ struct Geo {
     static constexpr int TPW = T >= TPB ? 1 : TPB / T;
     static constexpr int WG = T * TPW;
     static constexpr int NP = LOGN == 0 ? 0 : cdiv(LOGN, LOGR);
+    static constexpr bool LP2 = false;  // see GeoLP2
     // log2 radix of pass p: the first pass takes the remainder
     static constexpr int lr(int p) { return p == 0 ? LOGN - (NP - 1) * LOGR : LOGR; }
     // log2 width of pass p
@@ -59,6 +60,26 @@ struct Geo {
     // LDS row per transform: one pad slot per 16 complex values
     static constexpr int LDSN = N + (N >> 4) + 1;
     static constexpr int pad(int e) { return e + (e >> 4); }
+};
+
+// The same transform with its radix-2 remainder pass run last, between the
+// two halves of each wave (run_pass_img, v_permlane32_swap), instead of first
+// with an LDS exchange after it: one LDS exchange fewer.  Persistent streaming
+// kernels at 2^13 points (16 x 16 x 16 x 2).
+template <int LOGN, int LOGR_, int TPB>
+struct GeoLP2 : Geo<LOGN, LOGR_, TPB> {
+    using B = Geo<LOGN, LOGR_, TPB>;
+    static constexpr bool LP2 = true;
+    static_assert(B::NP >= 3 && LOGN - (B::NP - 1) * B::LOGR == 1 && B::T >= 64 && B::TPW == 1,
+                  "lane-pair radix 2: the remainder is 2 and a transform spans whole waves");
+    static constexpr int lr(int p) { return p == B::NP - 1 ? 1 : B::LOGR; }
+    static constexpr int lw(int p) { return B::LOGR * p; }
+    static constexpr int tw_off(int p) {
+        int o = 0;
+        for (int q = 1; q < p; ++q) o += ((1 << lr(q)) - 1) << lw(q);
+        return o;
+    }
+    static constexpr int tw_total() { return tw_off(B::NP); }
 };
 
 // ---------------------------------------------------------------- codelets
@@ -690,19 +711,39 @@ struct TwImage {
     static constexpr int total() { return G::NP > 1 ? off(G::NP) : 0; }
 };
 
+// Lane-pair radix 2 (G::LP2): the butterfly a thread runs in the pass before
+// the last, chosen so that butterflies i and i + T/2, whose outputs the
+// radix-2 pass combines, sit in lanes l and l + 32 of one wave.
+template <class G>
+__device__ __forceinline__ int lp2_index(int t) {
+    return (t & 31) + ((t >> 6) << 5) + (((t >> 5) & 1) ? G::T / 2 : 0);
+}
+
+// Output position of v[q] after the passes of a lane-pair geometry: that of
+// the pass before the last (the radix-2 pass keeps each lane's positions).
+template <class G>
+__device__ __forceinline__ int lp2_pos(int t, int q) {
+    constexpr int lw = G::lw(G::NP - 2), w = 1 << lw;
+    const int i = lp2_index<G>(t);
+    return ((i >> lw) << (lw + G::LOGR)) + (i & (w - 1)) + q * w;
+}
+
 // Pass P with twiddles from the LDS image `ltw` (see TwImage).
 template <class G, int P, int DMAX, class CX = float2>
 __device__ __forceinline__ void run_pass_img(CX (&v)[G::R], CX* slds, const CX* ltw, int t) {
     constexpr int lr = G::lr(P), r = 1 << lr, nb = G::R / r, lw = G::lw(P), w = 1 << lw;
     constexpr bool last = (P == G::NP - 1);
+    // the pass before a lane-pair radix 2: butterflies remapped (lp2_index)
+    constexpr bool lp2 = G::LP2 && P == G::NP - 2;
     using TI = TwImage<G, DMAX>;
+    static_assert(!lp2 || nb == 1, "lane-pair radix 2 after a full-radix pass");
 #pragma unroll
     for (int b = 0; b < nb; ++b) {
         CX x[r];
 #pragma unroll
         for (int m = 0; m < r; ++m) x[m] = v[b + m * nb];
         if constexpr (P > 0) {
-            const int i = t + G::T * b;
+            const int i = (lp2 ? lp2_index<G>(t) : t) + G::T * b;
             const int k = i & (w - 1);
             if constexpr (w <= DMAX) {
                 const CX* twp = ltw + TI::off(P) + k;
@@ -719,7 +760,30 @@ __device__ __forceinline__ void run_pass_img(CX (&v)[G::R], CX* slds, const CX* 
 #pragma unroll
         for (int q = 0; q < r; ++q) v[b + q * nb] = x[q];
     }
-    if constexpr (!last) {
+    if constexpr (lp2) {
+        // the last pass, radix 2 at width N/2: lane l (l < 32) holds y[k],
+        // lane l + 32 holds y[k + N/2], for k = lp2_pos(t, q); the lower lane
+        // keeps y[k] + W y[k + N/2], the upper one y[k] - W y[k + N/2], with
+        // W = W_N^k, both at their own positions
+        static_assert(sizeof(CX) == 8, "lane-pair radix 2: FP32");
+        constexpr int PL = G::NP - 1, wl = 1 << G::lw(PL);
+        const bool upper = (t >> 5) & 1;
+        const CX* lo = ltw + TI::off(PL);
+        const CX* hi = ltw + TI::off(PL) + TI::lo_n(PL);
+#pragma unroll
+        for (int q = 0; q < G::R; ++q) {
+            const auto sx = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[q].x), __float_as_uint(v[q].x),
+                                                             false, false);
+            const auto sy = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[q].y), __float_as_uint(v[q].y),
+                                                             false, false);
+            const CX a = CxT<CX>::mk(__uint_as_float(sx[0]), __uint_as_float(sy[0]));
+            const CX bb = CxT<CX>::mk(__uint_as_float(sx[1]), __uint_as_float(sy[1]));
+            const int k = lp2_pos<G>(t, q) & (wl - 1);
+            static_assert(wl > DMAX, "lane-pair radix 2: factored twiddles");
+            const CX wb = cmul(bb, cmul(lo[k & 15], hi[k >> 4]));
+            v[q] = upper ? csub(a, wb) : cadd(a, wb);
+        }
+    } else if constexpr (!last) {
         __syncthreads();
 #pragma unroll
         for (int b = 0; b < nb; ++b) {
@@ -729,8 +793,10 @@ __device__ __forceinline__ void run_pass_img(CX (&v)[G::R], CX* slds, const CX* 
             for (int q = 0; q < r; ++q) slds[G::pad(base + (q << lw))] = v[b + q * nb];
         }
         __syncthreads();
+        // the next pass's inputs x[i + (N / r') m] of its butterfly i
+        const int tn = (G::LP2 && P + 1 == G::NP - 2) ? lp2_index<G>(t) : t;
 #pragma unroll
-        for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
+        for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(tn + G::T * j)];
         run_pass_img<G, P + 1, DMAX, CX>(v, slds, ltw, t);
     }
 }
@@ -767,7 +833,6 @@ fft_stream_kernel(KArgs a) {
     __shared__ CX ltw[TI::total() > 0 ? TI::total() : 1];
     static_assert(!RTWG || KIND != C2C, "RFFT twiddles: real kinds only");
     __shared__ CX lrtw[KIND == C2C || RTWG ? 1 : (G::N / 2 > 0 ? G::N / 2 : 1)];
-    const CX* const grtw = static_cast<const CX*>(a.rtw);
     const int tid = threadIdx.x;
     const int slot = tid / G::T;
     const int t = tid - slot * G::T;
@@ -878,12 +943,19 @@ fft_stream_kernel(KArgs a) {
             }
         }
         if constexpr (G::NP > 0) run_pass_img<G, 0, DMAX>(v, slds, ltw, t);
+        // output position of v[j]: t + T j, or the lane-pair geometry's lp2_pos
         if constexpr (KIND == C2C) {
             void* o0 = active ? a.out0 : a.sink;
             void* o1 = active ? a.out1 : const_cast<void*>(sk1);
             const long long eo = active ? f * M : 0;
+            if constexpr (G::LP2) {
 #pragma unroll
-            for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, LOUT, eo + t + G::T * j, bwd ? swapri(v[j]) : v[j]);
+                for (int j = 0; j < G::R; ++j)
+                    st_c_nt(o0, o1, LOUT, eo + lp2_pos<G>(t, j), bwd ? swapri(v[j]) : v[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, LOUT, eo + t + G::T * j, bwd ? swapri(v[j]) : v[j]);
+            }
         } else {
             // post-twiddle epilogue, the arithmetic of fft_lds_kernel's.  A
             // frame's M + 1 outputs start off a cache-line boundary; with
@@ -899,8 +971,13 @@ fft_stream_kernel(KArgs a) {
             // the next group's wait for its prefetched input drain all of
             // these stores first.
             __syncthreads();
+            if constexpr (G::LP2) {
 #pragma unroll
-            for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
+                for (int j = 0; j < G::R; ++j) slds[G::pad(lp2_pos<G>(t, j))] = v[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < G::R; ++j) slds[G::pad(t + G::T * j)] = v[j];
+            }
             __syncthreads();
             const long long fb = active ? f * (M + 1) : 0;
             void* const o0 = active ? a.out0 : a.sink;
@@ -929,7 +1006,8 @@ fft_stream_kernel(KArgs a) {
                         const int k = 1 + tt + G::T * i;
                         const CX z1 = slds[G::pad(k)];
                         const CX z2 = slds[G::pad(M - k)];
-                        const CX w = RTWG ? grtw[k - 1] : lrtw[k - 1];
+                        CX w;
+                        if constexpr (RTWG) w = static_cast<const CX*>(a.rtw)[k - 1]; else w = lrtw[k - 1];
                         const S si = z1.y + z2.y, dr = z1.x - z2.x;
                         const CX xm = XT::mk(XT::fma(dr, w.y, XT::fma(-si, w.x, z2.x)),
                                              XT::fma(-dr, w.x, XT::fma(-si, w.y, z2.y)));

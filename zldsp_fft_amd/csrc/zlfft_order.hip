@@ -16,6 +16,12 @@
 #ifndef ZL_STREAM_LOGR
 #define ZL_STREAM_LOGR(n) ((n) == 6 ? 2 : (n) == 7 ? 3 : cfg_logr(n))
 #endif
+// The FP32 streaming variants at 2^13 points run the radix-2 remainder pass
+// last, between the halves of each wave (GeoLP2), saving one of the three
+// LDS exchanges.
+#ifndef ZL_STREAM_LP2
+#define ZL_STREAM_LP2 1
+#endif
 
 #define ZL_CAT2(a, b) a##b
 #define ZL_CAT(a, b) ZL_CAT2(a, b)
@@ -23,7 +29,12 @@
 void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
     using namespace zl;
     using G = Geo<ZL_LOGN, cfg_logr(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
-    using GS = Geo<ZL_LOGN, ZL_STREAM_LOGR(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
+    using GS0 = Geo<ZL_LOGN, ZL_STREAM_LOGR(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
+#if ZL_LOGN == 13 && ZL_STREAM_LP2
+    using GS = GeoLP2<ZL_LOGN, ZL_STREAM_LOGR(ZL_LOGN), cfg_tpb(ZL_LOGN)>;
+#else
+    using GS = GS0;
+#endif
     tab[C2C][AOS][AOS] = make_entry<G, C2C, AOS, AOS, GS>();
     tab[C2C][AOS][SOA] = make_entry<G, C2C, AOS, SOA, GS>();
     tab[C2C][SOA][AOS] = make_entry<G, C2C, SOA, AOS, GS>();
