@@ -393,7 +393,9 @@ def analyzer_lines(args) -> dict:
 
 def large_lines(args) -> dict:
     """The large SoA CFFTs: orders 16/18/20/22 at batch 64/16/4/1 (4,194,304
-    points per call), forward and backward, through the C-ABI."""
+    points per call), forward and backward, through the C-ABI; then the same
+    shapes as interleaved complex64, forward, through the C-ABI and through
+    rocFFT (torch.fft) for comparison."""
     import torch
     import oracle
     import zldsp_fft_amd as zf
@@ -421,8 +423,21 @@ def large_lines(args) -> dict:
             out[f"cfft_o{order}_b{batch}_soa_{name}"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall,
                                                               float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
                                                               kern)
+        # the same shape as interleaved complex64, here and through rocFFT
+        # (torch.fft, which takes no split layout): an external reference point
+        xa = [torch.complex(a, b).view(batch, n) for a, b in ins]
+        ya = [torch.empty_like(t) for t in xa]
+        fns = [lambda i=i: _call(lib, lib.zlfft_cfft_exec, h, zf.FORWARD, zf.AOS, xa[i].data_ptr(), None, zf.AOS,
+                                 ya[i].data_ptr(), None, batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        out[f"cfft_o{order}_b{batch}_aos_forward"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall, None,
+                                                           kern)
+        fns = [lambda i=i: torch.fft.fft(xa[i], dim=-1) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        out[f"rocfft_o{order}_b{batch}_aos_forward"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall,
+                                                             None, kern)
         plan.close()
-        del ins, outs
+        del ins, outs, xa, ya
         torch.cuda.empty_cache()
     return out
 

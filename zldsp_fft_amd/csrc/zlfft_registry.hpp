@@ -124,8 +124,10 @@ constexpr int kRaggedMaxOrderTune = 14;  // orders of the ragged tuning table
 // columns (butterflies) per workgroup of a radix-2^logl HBM pass: FP32, FP64
 // (C columns of 2^logl complex values, padded, fill the LDS)
 // (the widest that fits: fewer columns make each SoA access shorter than a
-// cache line, measured 8% slower at radix 256 and 35% at radix 512 / 1024)
-constexpr int cfg_macro_c(int logl) { return logl <= 8 ? 32 : 16; }
+// cache line, measured 8% slower at radix 256 and 35% at radix 512 / 1024;
+// radix 512 at 32 columns against 16: orders 17 and 19 on 4 Mi-point SoA
+// batches 3.5% and 4.4% faster)
+constexpr int cfg_macro_c(int logl) { return logl <= 9 ? 32 : 16; }
 constexpr int cfg_macro_c64(int logl) { return logl <= 8 ? 32 : logl == 9 ? 16 : 8; }
 
 // radix (log2) of each HBM pass for a CFFT order above kMaxLdsOrder, first pass first
