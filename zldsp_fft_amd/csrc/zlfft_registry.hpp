@@ -9,8 +9,12 @@ constexpr int kMaxLdsOrder = 14;    // largest CFFT order kept whole in LDS (FP3
 constexpr int kMaxLdsOrder64 = 13;  // the same for FP64 (2^14 double2 exceed the LDS)
 
 // Points-per-thread (log2) and target workgroup size of the LDS kernel for a
-// CFFT order.  Orders <= 4 keep a whole transform in one thread.
-constexpr int cfg_logr(int logn) { return logn <= 4 ? logn : 4; }
+// CFFT order.  Orders <= 3 keep a whole transform in one thread; orders 4 and
+// 5 run 8 points per thread: a 256-thread workgroup then stages half as many
+// points through LDS, and twice as many workgroups fit a CU (measured, 2^26
+// points per call, against 16 per thread: CFFT order 4 4523 -> 5609 GB/s,
+// order 5 3958 -> 5622; RFFT 2^6 bins 3076 -> 4555).
+constexpr int cfg_logr(int logn) { return logn <= 3 ? logn : logn <= 5 ? 3 : 4; }
 constexpr int cfg_tpb(int) { return 256; }
 
 struct KernelEntry {
