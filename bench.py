@@ -444,8 +444,8 @@ def large_lines(args) -> dict:
 
 def order_lines(args) -> dict:
     """The plain batched path by size (not part of the default run): CFFT
-    orders 3-14 (AoS forward) and RFFT orders 4-15 (AoS bins and squared
-    magnitudes), 2^26 points per call, through the C-ABI over ROTATE buffer
+    orders 3-14 (AoS forward) and RFFT orders 4-15 (AoS bins, squared
+    magnitudes, and backward from AoS bins), 2^26 points per call, through the C-ABI over ROTATE buffer
     sets; the persistent streaming kernel runs wherever the order has one."""
     import torch
     import zldsp_fft_amd as zf
@@ -483,6 +483,10 @@ def order_lines(args) -> dict:
                                  batch, sp) for i in range(ROTATE)]
         wall, kern = measure_line(torch, stream, fns)
         out[f"rfft_o{order + 1}_sqr_mag"] = _line((4 * rn + 4 * nb) * batch, flops, batch, wall, None, kern)
+        fns = [lambda i=i: _call(lib, lib.zlfft_rfft_backward_exec, h, zf.AOS, bins[i].data_ptr(), None, ins[i].data_ptr(),
+                                 batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        out[f"rfft_o{order + 1}_backward"] = _line((8 * nb + 4 * rn) * batch, flops, batch, wall, None, kern)
         plan.close()
         del ins, bins, sq
         torch.cuda.empty_cache()

@@ -22,6 +22,14 @@
 #ifndef ZL_STREAM_LP2
 #define ZL_STREAM_LP2 1
 #endif
+// The FP32 C2C streaming variants at 2^14 points run 32 points per thread:
+// 16 x 32 x 32, two LDS exchanges instead of three, 512 threads at 256 VGPRs
+// (measured, 2^26-point AoS batches: 4345 -> 4770 GB/s forward).  The real
+// kinds keep 16 points per thread: they share one twiddle image, and the
+// squared-magnitude variant spills at 32 (6% slower there).
+#ifndef ZL_STREAM_C2C_R32
+#define ZL_STREAM_C2C_R32 1
+#endif
 
 #define ZL_CAT2(a, b) a##b
 #define ZL_CAT(a, b) ZL_CAT2(a, b)
@@ -35,10 +43,15 @@ void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
 #else
     using GS = GS0;
 #endif
-    tab[C2C][AOS][AOS] = make_entry<G, C2C, AOS, AOS, GS>();
-    tab[C2C][AOS][SOA] = make_entry<G, C2C, AOS, SOA, GS>();
-    tab[C2C][SOA][AOS] = make_entry<G, C2C, SOA, AOS, GS>();
-    tab[C2C][SOA][SOA] = make_entry<G, C2C, SOA, SOA, GS>();
+#if ZL_LOGN == 14 && ZL_STREAM_C2C_R32
+    using GC = Geo<ZL_LOGN, 5, cfg_tpb(ZL_LOGN)>;
+#else
+    using GC = GS;
+#endif
+    tab[C2C][AOS][AOS] = make_entry<G, C2C, AOS, AOS, GC>();
+    tab[C2C][AOS][SOA] = make_entry<G, C2C, AOS, SOA, GC>();
+    tab[C2C][SOA][AOS] = make_entry<G, C2C, SOA, AOS, GC>();
+    tab[C2C][SOA][SOA] = make_entry<G, C2C, SOA, SOA, GC>();
     tab[R2C][AOS][AOS] = make_entry<G, R2C, AOS, AOS, GS>();
     tab[R2C][AOS][SOA] = make_entry<G, R2C, AOS, SOA, GS>();
     tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS, GS>();
