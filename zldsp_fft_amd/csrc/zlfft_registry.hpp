@@ -185,7 +185,8 @@ using MacroTable = MacroEntry[3][3];
 
 template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1>
 MacroEntry make_macro_entry() {
-    static_assert((long long)sizeof(CX) * C * GL::LDSN <= 160 * 1024, "HBM pass exceeds the LDS");
+    static_assert((long long)sizeof(CX) * (C * GL::LDSN + (GL::logn <= 7 ? GL::tw_total() : 0)) <= 160 * 1024,
+                  "HBM pass exceeds the LDS");
     MacroEntry e{};
     e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT, CX, NT>;
     e.wg = C * GL::T;
