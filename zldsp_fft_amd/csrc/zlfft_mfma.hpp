@@ -68,32 +68,49 @@ __device__ __forceinline__ float bf16_trunc(float x) {
 
 typedef float zl_fv2 __attribute__((ext_vector_type(2)));
 
-// FP16 split of data: hi = v with the low 13 mantissa bits cleared (v
-// truncated to 11 significant bits, an FP16 value when v is in FP16's normal
-// range, so its conversion is exact) and lo = v - hi (exact), rounded to FP16
-// by packed conversions.  Integer and plain FP32 operations only: the
-// mixed-precision FMAs (v_fma_mix*) of the round-to-nearest form measured
-// slower on gfx950.  Unsplit (NS = 1): v rounded to nearest.
+// FP16 split of data: hi = v truncated to 11 significant bits (the
+// round-toward-zero packed conversion v_cvt_pkrtz_f16_f32, exact truncation in
+// FP16's normal range) and lo = v - hi, exact in FP32 (one v_fma_mix_f32 with
+// hi as its FP16 operand), rounded to FP16 by a packed conversion: two VALU
+// operations per value instead of three for the form that masks the low 13
+// mantissa bits and converts both parts (the same parts in FP16's normal
+// range; measured on the 16384 x 4096 batch: 13% fewer VALU instructions,
+// 0.5% faster).  Unsplit (NS = 1): v rounded to nearest.
+// v - (float)h for the FP16 value h in the low (HI = 0) or high (HI = 1) half
+// of hp: one v_fma_mix_f32 (FP16 operand converted inside the FMA, exact here)
+template <int HI>
+__device__ __forceinline__ float sub_f16(unsigned hp, float v) {
+    float r;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(hp), "v"(v));
+    else
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[0,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(hp), "v"(v));
+    return r;
+}
+
 template <int NS>
-__device__ __forceinline__ void split8_mask(const float* v, zl_h8 (&parts)[NS]) {
+__device__ __forceinline__ void split8_trunc(const float* v, zl_h8 (&parts)[NS]) {
 #pragma unroll
     for (int e = 0; e < 8; e += 2) {
-        const float h0 = __uint_as_float(__float_as_uint(v[e]) & 0xffffe000u);
-        const float h1 = __uint_as_float(__float_as_uint(v[e + 1]) & 0xffffe000u);
-        const zl_h2 hp = __builtin_convertvector((zl_fv2){NS == 2 ? h0 : v[e], NS == 2 ? h1 : v[e + 1]}, zl_h2);
-        parts[0][e] = hp.x;
-        parts[0][e + 1] = hp.y;
         if constexpr (NS == 2) {
-            const zl_h2 lp = __builtin_convertvector((zl_fv2){v[e] - h0, v[e + 1] - h1}, zl_h2);
+            const auto hp = __builtin_amdgcn_cvt_pkrtz(v[e], v[e + 1]);
+            const unsigned hb = __builtin_bit_cast(unsigned, hp);
+            parts[0][e] = hp[0];
+            parts[0][e + 1] = hp[1];
+            const zl_h2 lp = __builtin_convertvector((zl_fv2){sub_f16<0>(hb, v[e]), sub_f16<1>(hb, v[e + 1])}, zl_h2);
             parts[1][e] = lp.x;
             parts[1][e + 1] = lp.y;
+        } else {
+            const zl_h2 hp = __builtin_convertvector((zl_fv2){v[e], v[e + 1]}, zl_h2);
+            parts[0][e] = hp.x;
+            parts[0][e + 1] = hp.y;
         }
     }
 }
 
 // Splits 8 FP32 values into NS parts of the MFMA input type, v = sum of the
 // parts to about NS x (mantissa bits) bits.
-//  FP16: data (RNE = false) as split8_mask; constants (RNE = true) with
+//  FP16: data (RNE = false) as split8_trunc; constants (RNE = true) with
 //        round-to-nearest parts, each the FP16 rounding of the previous
 //        remainder (exact in FP32).
 //  BF16: with three parts, the first two are truncations of the remainder to
@@ -111,7 +128,7 @@ __device__ __forceinline__ void split8(const float* v, typename MfOp<P>::V (&par
                 if constexpr (NS == 2) parts[1][e] = (_Float16)(v[e] - (float)h);
             }
         } else {
-            split8_mask<NS>(v, parts);
+            split8_trunc<NS>(v, parts);
         }
     } else {
 #pragma unroll
@@ -145,7 +162,7 @@ __device__ __forceinline__ void split8_scaled(const float* v, float s, zl_h8 (&p
     float w[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) w[e] = v[e] * s;
-    split8_mask<NS>(w, parts);
+    split8_trunc<NS>(w, parts);
 }
 
 // cos(2 pi j / 16), j = 0..15, in double
