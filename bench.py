@@ -70,6 +70,8 @@ def parse():
                          "the plain batched path by size (orders: only this way, not in the default run)")
     ap.add_argument("--mfma-precisions", default=None,
                     help="comma-separated subset of the MFMA variant names")
+    ap.add_argument("--no-copy-ref", action="store_true",
+                    help="skip the plain-copy reference rates beside the lines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 512 MiB device copy (reference kernel for profiler byte counters)")
     ap.add_argument("--warm-seconds", type=float, default=WARM_S,
@@ -125,7 +127,7 @@ def time_each(torch, stream, fns, reps):
 
 def _line(nbytes, flops, count, ms, rel=None, kern_ms=None):
     gbs = nbytes / (ms * 1e-3) / 1e9
-    d = {"ms": round(ms, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+    d = {"ms": round(ms, 5), "bytes": int(nbytes), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
          "transforms_per_s": round(count / (ms * 1e-3), 1), "gflops": round(flops / (ms * 1e-3) / 1e9, 1)}
     if kern_ms is not None:
         kg = nbytes / (kern_ms * 1e-3) / 1e9
@@ -556,6 +558,69 @@ def rocfft_c4(x, stream, args) -> dict:
     return _line(BYTES_PER_STEP, 5.0 * N * ORDER * BATCH, BATCH, wall, None, kern)
 
 
+def copy_reference(nbytes: int, structure: int = 0, per_cu: int = 2) -> dict:
+    """A plain device copy of the same algorithmic traffic (nbytes / 2 read,
+    nbytes / 2 written), timed like the lines (warm-up by time, back-to-back
+    launches over ROTATE buffer sets): the practical ceiling of a memory-bound
+    kernel that moves those bytes in one launch.  structure 0: one 32 KiB chunk
+    per workgroup; 1: the persistent streaming kernels' form, per_cu resident
+    256-thread workgroups per CU (libzlprobe.so)."""
+    import ctypes
+    import torch
+    lib = _probe_lib()
+    chunk = lib.zlprobe_chunk_bytes()
+    half = max(chunk, (nbytes // 2) // chunk * chunk)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    src = [torch.empty(half // 4, device="cuda") for _ in range(ROTATE)]
+    dst = [torch.empty(half // 4, device="cuda") for _ in range(ROTATE)]
+    for t in src:
+        t.uniform_()
+
+    def run(i):
+        if lib.zlprobe_copy(ctypes.c_void_p(src[i].data_ptr()), ctypes.c_void_p(dst[i].data_ptr()),
+                            ctypes.c_size_t(half), structure, per_cu, ctypes.c_void_p(sp)):
+            raise RuntimeError("zlprobe_copy failed")
+    wall, _ = measure_line(torch, stream, [lambda i=i: run(i) for i in range(ROTATE)])
+    if not torch.equal(src[0], dst[0]):
+        raise RuntimeError("reference copy is wrong")
+    del src, dst
+    torch.cuda.empty_cache()
+    gbs = 2 * half / (wall * 1e-3) / 1e9
+    return {"bytes": 2 * half, "ms": round(wall, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+_PROBE = None
+
+
+def _probe_lib():
+    global _PROBE
+    if _PROBE is None:
+        import ctypes
+        path = os.path.join(ROOT, "zldsp_fft_amd", "libzlprobe.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+        _PROBE = ctypes.CDLL(path)
+        _PROBE.zlprobe_chunk_bytes.restype = ctypes.c_size_t
+    return _PROBE
+
+
+def annotate_with_copies(lines: dict, cache: dict) -> None:
+    """Adds to every line with a byte count the rate of a copy of the same
+    bytes (copy_GB/s) and the line's fraction of it (of_copy)."""
+    for name, d in lines.items():
+        if not isinstance(d, dict) or "GB/s" not in d or name.startswith(("host_", "rocfft")):
+            continue
+        nbytes = d.get("bytes")
+        if not nbytes:
+            continue
+        if nbytes not in cache:
+            cache[nbytes] = copy_reference(nbytes)
+        c = cache[nbytes]
+        d["copy_GB/s"] = c["GB/s"]
+        d["of_copy"] = round(d["GB/s"] / c["GB/s"], 4)
+
+
 def traffic_bytes():
     """HBM bytes per launch measured with rocprofv3 PMC counters (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel of known size), if profiled."""
@@ -594,7 +659,10 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.only in sections:
-        print(json.dumps({"section": args.only, args.only: sections[args.only](args)}), flush=True)
+        res = sections[args.only](args)
+        if args.only != "ragged" and not args.no_copy_ref:
+            annotate_with_copies(res, {})
+        print(json.dumps({"section": args.only, args.only: res}), flush=True)
         return
     dev = torch.device("cuda:0")
     plan = zf.CFFT(ORDER)
@@ -671,13 +739,28 @@ def main():
         out["mfma"] = mfma_variants(x, y, stream, lib, args, step)
     if args.only != "main":
         out["rocfft"] = rocfft_c4(x, stream, args)
+    copies = {}
+    if not args.no_copy_ref:
+        # the same 1 GiB moved by a plain copy in one launch, in the fastest
+        # form measured and in the streaming kernel's form
+        grid = copy_reference(BYTES_PER_STEP, 0)
+        copies[BYTES_PER_STEP] = grid
+        out["copy_reference"] = {"one_chunk_per_workgroup": grid,
+                                 "streaming_form_2_per_cu": copy_reference(BYTES_PER_STEP, 1, 2),
+                                 "value_of_copy": round(gbs / grid["GB/s"], 4)}
     if not args.no_configs:
         del x, y
         torch.cuda.empty_cache()
         out["configs"] = config_lines(args)
+        if not args.no_copy_ref:
+            annotate_with_copies(out["configs"], copies)
     if not args.no_ragged:
         torch.cuda.empty_cache()
         out["ragged"] = ragged_section(args)
+        if not args.no_copy_ref:
+            c = copy_reference(out["ragged"]["bytes_per_launch"])
+            out["ragged"]["copy_GB/s"] = c["GB/s"]
+            out["ragged"]["of_copy"] = round(out["ragged"]["GB/s"] / c["GB/s"], 4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     if rel > 1e-6:

@@ -100,3 +100,21 @@ def test_ragged_desc_numpy_image_matches_struct():
     assert d.itemsize == ctypes.sizeof(zf.RaggedDesc) == 48
     for name, _ in zf.RaggedDesc._fields_:
         assert d.fields[name][1] == getattr(zf.RaggedDesc, name).offset, name
+
+
+def test_copy_reference_library_exports():
+    """libzlprobe.so (the plain-copy reference that bench.py times beside its
+    lines) loads without a GPU and exports its two functions, nothing else;
+    invalid arguments are rejected before any launch."""
+    import subprocess
+    path = os.path.join(ROOT, "zldsp_fft_amd", "libzlprobe.so")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    syms = {line.split()[-1] for line in out.splitlines() if len(line.split()) >= 3}
+    assert syms == {"zlprobe_copy", "zlprobe_chunk_bytes"}, syms
+    lib = ctypes.CDLL(path)
+    lib.zlprobe_chunk_bytes.restype = ctypes.c_size_t
+    assert lib.zlprobe_chunk_bytes() == 32768
+    assert lib.zlprobe_copy(None, None, ctypes.c_size_t(1 << 20), 0, 2, None) == -1
+    buf = ctypes.create_string_buffer(64)
+    assert lib.zlprobe_copy(buf, buf, ctypes.c_size_t(100), 0, 2, None) == -1  # less than one chunk
+    assert lib.zlprobe_copy(buf, buf, ctypes.c_size_t(1 << 20), 7, 2, None) == -1  # unknown structure

@@ -132,9 +132,29 @@ def build(jobs: int | None = None, verbose: bool = False) -> str:
         print(err, file=sys.stderr)
         raise RuntimeError("link of libzlfft.so failed")
     os.replace(tmp, LIB)
+    build_probe()
     build_dropin_example()
     link_sanitized()
     return LIB
+
+
+PROBE_SRC = os.path.join(CSRC, "zlfft_probe.hip")
+PROBE_LIB = os.path.join(HERE, "libzlprobe.so")
+
+
+def build_probe() -> str:
+    """libzlprobe.so: the device-copy reference kernels bench.py times beside
+    each line (not part of the transform library)."""
+    tmp = PROBE_LIB + ".tmp"
+    vmap = os.path.join(OBJ, "probe_exports.map")
+    with open(vmap, "w") as f:
+        f.write("{ global: zlprobe_*; local: *; };\n")
+    _, rc, err = _run([HIPCC, *HIP_FLAGS, "-shared", f"-Wl,--version-script={vmap}", PROBE_SRC, "-o", tmp])
+    if rc:
+        print(err, file=sys.stderr)
+        raise RuntimeError("build of libzlprobe.so failed")
+    os.replace(tmp, PROBE_LIB)
+    return PROBE_LIB
 
 
 DROPIN_SRC = os.path.join(ROOT, "tests", "dropin", "dropin_example.cpp")
