@@ -393,6 +393,12 @@ def analyzer_lines(args) -> dict:
     return out
 
 
+def hbm_passes(order: int) -> int:
+    """HBM passes of the FP32 CFFT of an order above 14 (2 up to order 20,
+    then 3): the order's multi-pass plan in zldsp_fft_amd/csrc/zlfft_registry.hpp."""
+    return 2 if order <= 20 else 3
+
+
 def large_lines(args) -> dict:
     """The large SoA CFFTs: orders 16/18/20/22 at batch 64/16/4/1 (4,194,304
     points per call), forward and backward, through the C-ABI; then the same
@@ -425,6 +431,7 @@ def large_lines(args) -> dict:
             out[f"cfft_o{order}_b{batch}_soa_{name}"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall,
                                                               float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
                                                               kern)
+            out[f"cfft_o{order}_b{batch}_soa_{name}"]["hbm_passes"] = hbm_passes(order)
         # the same shape as interleaved complex64, here and through rocFFT
         # (torch.fft, which takes no split layout): an external reference point
         xa = [torch.complex(a, b).view(batch, n) for a, b in ins]
@@ -434,6 +441,7 @@ def large_lines(args) -> dict:
         wall, kern = measure_line(torch, stream, fns)
         out[f"cfft_o{order}_b{batch}_aos_forward"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall, None,
                                                            kern)
+        out[f"cfft_o{order}_b{batch}_aos_forward"]["hbm_passes"] = hbm_passes(order)
         fns = [lambda i=i: torch.fft.fft(xa[i], dim=-1) for i in range(ROTATE)]
         wall, kern = measure_line(torch, stream, fns)
         out[f"rocfft_o{order}_b{batch}_aos_forward"] = _line(16 * n * batch, 5.0 * n * order * batch, batch, wall,
@@ -619,6 +627,9 @@ def annotate_with_copies(lines: dict, cache: dict) -> None:
         c = cache[nbytes]
         d["copy_GB/s"] = c["GB/s"]
         d["of_copy"] = round(d["GB/s"] / c["GB/s"], 4)
+        if d.get("hbm_passes", 1) > 1:
+            # a k-pass transform moves its bytes k times: k copies are its ceiling
+            d["of_pass_copies"] = round(d["GB/s"] * d["hbm_passes"] / c["GB/s"], 4)
 
 
 def traffic_bytes():
