@@ -376,6 +376,19 @@ def analyzer_lines(args) -> dict:
     rsq = np.abs(ref) ** 2
     rel = float((np.linalg.norm(got - rsq, axis=1) / np.linalg.norm(rsq, axis=1)).max())
     out["rfft2048_b4096_sqr_mag"] = _line(frames * (4 * n + 4 * nb), flops, frames, wall, rel, kern)
+    # RFFT<float>::backward over the batch: the bins just computed back to
+    # frames (N/2 x the frames), AoS bins in
+    rs = [torch.empty(frames * n, dtype=torch.float32, device=dev) for _ in range(ROTATE)]
+    bw = [lambda i=i: _call(lib, lib.zlfft_rfft_backward_exec, h, zf.AOS, ys[i].data_ptr(), None, rs[i].data_ptr(),
+                            frames, sp) for i in range(ROTATE)]
+    for f in fwd:
+        f()
+    wall, kern = measure_line(torch, stream, bw)
+    binsd = ys[0].view(frames, nb)[[7, 2048, 4095]].cpu().numpy().astype(np.complex128)
+    refb = np.stack([oracle.rfft_backward(b, n, "f64") for b in binsd])
+    gotb = rs[0].view(frames, n)[[7, 2048, 4095]].cpu().numpy().astype(np.float64)
+    relb = float((np.linalg.norm(gotb - refb, axis=1) / np.linalg.norm(refb, axis=1)).max())
+    out["rfft2048_b4096_backward"] = _line(frames * (8 * nb + 4 * n), flops, frames, wall, relb, kern)
     wall, kern = measure_line(torch, stream, [lambda i=i: torch.fft.rfft(xs[i], dim=-1) for i in range(ROTATE)])
     out["rocfft_rfft2048_b4096_aos"] = _line(frames * (4 * n + 8 * nb), flops, frames, wall, None, kern)
     # STFT front-end: 4096 Hann-windowed frames of 2048 samples at hop 512 of

@@ -318,3 +318,28 @@ def test_streaming_batches(order):
         r.forward_sqr_mag(xr[lo:hi], Ss)
         torch.cuda.synchronize()
         assert torch.equal(Ys, Y[lo:hi]) and torch.equal(Ss, S[lo:hi]), (order, lo)
+    # backward (C2R) over the same batch, from random bins whose DC and
+    # Nyquist imaginary parts must be ignored: AoS and SoA bins give the same
+    # bits, sampled frames match the FP64 oracle, and (where the one-launch
+    # kernel has the same geometry) small batches give the same bits
+    Xb = torch.randn((batch, n + 1), dtype=torch.complex64, device="cuda", generator=g)
+    xb = torch.empty((batch, 2 * n), dtype=torch.float32, device="cuda")
+    xb2 = torch.empty_like(xb)
+    r.backward(Xb, xb)
+    r.backward((Xb.real.contiguous(), Xb.imag.contiguous()), xb2)
+    torch.cuda.synchronize()
+    # at orders 6-7 SoA bins (16 threads x 4 bytes per row) take the one-launch
+    # kernel at 16 points per thread, AoS bins the streaming one at 4 / 8
+    if order not in (6, 7):
+        assert torch.equal(xb, xb2), order
+    bins = Xb[idx].cpu().numpy().astype(np.complex128)
+    refb = np.stack([oracle.rfft_backward(b, 2 * n, "f64") for b in bins])
+    for out in (xb, xb2):
+        eb = rel(out[idx].cpu().numpy().astype(np.float64), refb).max()
+        print(f"stream rfft backward order={order + 1} batch={batch} relL2={eb:.3e}", flush=True)
+        assert eb <= TOL
+    for lo, hi in ((0, 8), (batch - 11, batch)) if order <= 8 and order not in (6, 7) else ():
+        xs_ = torch.empty((hi - lo, 2 * n), dtype=torch.float32, device="cuda")
+        r.backward(Xb[lo:hi], xs_)
+        torch.cuda.synchronize()
+        assert torch.equal(xs_, xb[lo:hi]), (order, lo)

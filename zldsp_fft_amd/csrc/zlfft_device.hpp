@@ -822,7 +822,8 @@ __device__ __forceinline__ void run_pass_img(CX (&v)[G::R], CX* slds, const CX* 
 template <class G, int KIND, int LIN, int LOUT, int BWD, int FRAMED = 0, class CX = float2, int RTWG = 0>
 __global__ void __launch_bounds__(G::WG) __attribute__((amdgpu_waves_per_eu(ZL_STREAM_MIN_WAVES, 8)))
 fft_stream_kernel(KArgs a) {
-    static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR, "streaming kernel handles C2C and R2C");
+    static_assert(KIND == C2C || KIND == R2C || KIND == R2C_SQR || KIND == C2R,
+                  "streaming kernel handles C2C, R2C and C2R");
     static_assert(!FRAMED || KIND != C2C, "framed input is real");
     static_assert(sizeof(CX) == 8 || !FRAMED, "FP64 streaming kernel: not framed");
     using XT = CxT<CX>;
@@ -841,6 +842,7 @@ fft_stream_kernel(KArgs a) {
     constexpr int lin = (KIND == C2C) ? LIN : AOS;
     constexpr bool bwd = (KIND == C2C) && BWD != 0;  // direction fixed per instantiation
     const long long ngroups = (a.batch + G::TPW - 1) / G::TPW;
+    static_assert(KIND != C2R || (!FRAMED && G::R >= 2), "C2R: plain frames of whole bin pairs");
 
     // LDS tables (twiddle image, RFFT twiddles, STFT window): all their loads
     // are issued first, then the first group's prefetch, then the LDS writes,
@@ -879,8 +881,25 @@ fft_stream_kernel(KArgs a) {
     const void* sk1 = reinterpret_cast<const float*>(a.sink) + (1 << 14);
     // prefetch of frame / transform f into p (the sink where !ok); a framed
     // lane reads floats [0, 2 M) of the sink, M <= 2^14 (the sink holds 2^16)
+    // C2R: a frame is M + 1 bins in layout LIN; lane t prefetches the bin
+    // pairs (k, M - k), k = 1 + t + T i for i < R / 2 (each k in 1..M/2 once),
+    // into p[i] and p[R / 2 + i], and bins 0 and M into pe (every lane of the
+    // frame alike)
+    CX pe[KIND == C2R ? 2 : 1];
     auto prefetch = [&](CX (&p)[G::R], long long f, bool ok) {
-        if constexpr (FRAMED) {
+        if constexpr (KIND == C2R) {
+            const void* b0 = ok ? a.in0 : sk0;
+            const void* b1 = ok ? a.in1 : sk1;
+            const long long eo = ok ? f * (M + 1) : 0;
+#pragma unroll
+            for (int i = 0; i < G::R / 2; ++i) {
+                const int k = 1 + t + G::T * i;
+                p[i] = ld_c_nt<CX>(b0, b1, LIN, eo + k);
+                p[G::R / 2 + i] = ld_c_nt<CX>(b0, b1, LIN, eo + (M - k));
+            }
+            pe[0] = ld_c_nt<CX>(b0, b1, LIN, eo);
+            pe[1] = ld_c_nt<CX>(b0, b1, LIN, eo + M);
+        } else if constexpr (FRAMED) {
             const float* x = static_cast<const float*>(ok ? a.in0 : sk0) + (ok ? f * a.hop : 0);
 #pragma unroll
             for (int j = 0; j < G::R; ++j) {
@@ -928,13 +947,48 @@ fft_stream_kernel(KArgs a) {
         const long long f = g * G::TPW + slot;
         const bool active = f < a.batch;
         CX v[G::R];
+        CX u[KIND == C2R ? G::R : 1], ue[KIND == C2R ? 2 : 1];
+        if constexpr (KIND == C2R) {
 #pragma unroll
-        for (int j = 0; j < G::R; ++j) v[j] = bwd ? swapri(p[j]) : p[j];
+            for (int j = 0; j < G::R; ++j) u[j] = p[j];
+            ue[0] = pe[0];
+            ue[1] = pe[1];
+        } else {
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) v[j] = bwd ? swapri(p[j]) : p[j];
+        }
         {
             const long long fn = (g + gridDim.x) * G::TPW + slot;
             prefetch(p, fn, g + gridDim.x < ngroups && fn < a.batch);
         }
         __syncthreads();  // LDS image ready (first group) / previous group's LDS readers done
+        if constexpr (KIND == C2R) {
+            // pre-twiddle prologue of fft_lds_kernel (reference
+            // src/common/zldsp_fft_common_execute.hpp:398-468) on the
+            // prefetched bin pairs: Z written to LDS already swapped for the
+            // backward CFFT; thread 0 of the frame writes Z[0] from bins 0, M
+            const CX* rt = RTWG ? static_cast<const CX*>(a.rtw) : lrtw;
+            if (t == 0)
+                slds[0] = swapri(XT::mk((ue[0].x + ue[1].x) * S(0.5), (ue[0].x - ue[1].x) * S(0.5)));
+#pragma unroll
+            for (int i = 0; i < G::R / 2; ++i) {
+                const int k = 1 + t + G::T * i;
+                const CX x1 = u[i], x2 = u[G::R / 2 + i];
+                const CX w = rt[k - 1];  // (wc, ws)
+                const S si = x1.y + x2.y, dr = x1.x - x2.x;
+                const S zr_m = XT::fma(dr, w.y, XT::fma(si, w.x, x2.x));
+                const S zi_m = XT::fma(dr, w.x, XT::fma(-si, w.y, x2.y));
+                if (k != M - k) {
+                    const S zr_k = XT::fma(-dr, w.y, XT::fma(-si, w.x, x1.x));
+                    const S zi_k = XT::fma(dr, w.x, XT::fma(-si, w.y, x1.y));
+                    slds[G::pad(k)] = XT::mk(zi_k, zr_k);
+                }
+                slds[G::pad(M - k)] = XT::mk(zi_m, zr_m);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < G::R; ++j) v[j] = slds[G::pad(t + G::T * j)];
+        }
         if constexpr (FRAMED) {
 #pragma unroll
             for (int j = 0; j < G::R; ++j) {
@@ -943,18 +997,21 @@ fft_stream_kernel(KArgs a) {
             }
         }
         if constexpr (G::NP > 0) run_pass_img<G, 0, DMAX>(v, slds, ltw, t);
-        // output position of v[j]: t + T j, or the lane-pair geometry's lp2_pos
-        if constexpr (KIND == C2C) {
+        // output position of v[j]: t + T j, or the lane-pair geometry's lp2_pos;
+        // C2R writes its N reals as M pairs, swapped back from the backward CFFT
+        if constexpr (KIND == C2C || KIND == C2R) {
+            constexpr int lout = KIND == C2C ? LOUT : AOS;
+            constexpr bool sw = bwd || KIND == C2R;
             void* o0 = active ? a.out0 : a.sink;
             void* o1 = active ? a.out1 : const_cast<void*>(sk1);
             const long long eo = active ? f * M : 0;
             if constexpr (G::LP2) {
 #pragma unroll
                 for (int j = 0; j < G::R; ++j)
-                    st_c_nt(o0, o1, LOUT, eo + lp2_pos<G>(t, j), bwd ? swapri(v[j]) : v[j]);
+                    st_c_nt(o0, o1, lout, eo + lp2_pos<G>(t, j), sw ? swapri(v[j]) : v[j]);
             } else {
 #pragma unroll
-                for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, LOUT, eo + t + G::T * j, bwd ? swapri(v[j]) : v[j]);
+                for (int j = 0; j < G::R; ++j) st_c_nt(o0, o1, lout, eo + t + G::T * j, sw ? swapri(v[j]) : v[j]);
             }
         } else {
             // post-twiddle epilogue, the arithmetic of fft_lds_kernel's.  A

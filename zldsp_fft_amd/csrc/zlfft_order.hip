@@ -55,8 +55,8 @@ void ZL_CAT(zl_fill_order_, ZL_LOGN)(zl::OrderTable& tab) {
     tab[R2C][AOS][AOS] = make_entry<G, R2C, AOS, AOS, GS>();
     tab[R2C][AOS][SOA] = make_entry<G, R2C, AOS, SOA, GS>();
     tab[R2C_SQR][AOS][AOS] = make_entry<G, R2C_SQR, AOS, AOS, GS>();
-    tab[C2R][AOS][AOS] = make_entry<G, C2R, AOS, AOS>();
-    tab[C2R][SOA][AOS] = make_entry<G, C2R, SOA, AOS>();
+    tab[C2R][AOS][AOS] = make_entry<G, C2R, AOS, AOS, GS>();
+    tab[C2R][SOA][AOS] = make_entry<G, C2R, SOA, AOS, GS>();
     tab[R2C_FRAMED][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, float2, 1>();
     tab[R2C_FRAMED][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA, float2, 1>();
     tab[R2C_SQR_FRAMED][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS, float2, 1>();

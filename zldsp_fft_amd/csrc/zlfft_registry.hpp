@@ -51,7 +51,7 @@ using OrderTable = KernelEntry[6][2][2];
 template <class G, int KIND, int LIN, int LOUT, int FRAMED = 0, class CX = float2>
 void add_stream_variant(KernelEntry& e) {
     constexpr int cs = (int)sizeof(CX);
-    constexpr int elem = (KIND == C2C && (LIN == SOA || LOUT == SOA)) ? cs / 2 : cs;
+    constexpr int elem = ((KIND == C2C && (LIN == SOA || LOUT == SOA)) || (KIND == C2R && LIN == SOA)) ? cs / 2 : cs;
     constexpr bool direct = G::T * elem >= 128 || G::TPW == 1;
     using TI = TwImage<G, cfg_stream_direct_max<CX>(G::logn, KIND)>;
     // FP32 forward real kinds of inner order >= ZL_RTW_GLOBAL_MIN read the
@@ -60,9 +60,14 @@ void add_stream_variant(KernelEntry& e) {
     constexpr long long stream_lds =
         (long long)cs * (G::TPW * G::LDSN + TI::total() + (KIND == C2C || rtwg ? 1 : G::N / 2) + (FRAMED ? G::N : 0));
     constexpr bool fits = cs == 8 ? stream_lds <= 160 * 1024 : !FRAMED && 2 * stream_lds <= 160 * 1024;
-    if constexpr (KIND != C2R && direct && G::NP >= 2 && fits) {
+    // C2R holds the frame's bin pairs and bins 0 and M besides the points:
+    // at 1024 threads (128 VGPRs) it spilled, so those sizes keep the
+    // one-launch kernel
+    constexpr bool regs_ok = KIND != C2R || G::WG <= 512;
+    if constexpr (direct && G::NP >= 2 && fits && regs_ok) {
         e.stream_fn[0] = fft_stream_kernel<G, KIND, LIN, LOUT, 0, FRAMED, CX, rtwg ? 1 : 0>;
         if constexpr (KIND == C2C) e.stream_fn[1] = fft_stream_kernel<G, KIND, LIN, LOUT, 1, 0, CX>;
+        if constexpr (KIND == C2R) e.stream_fn[1] = e.stream_fn[0];  // the direction is the kind's
         e.s_wg = G::WG;
         e.s_tpw = G::TPW;
         e.s_np = G::NP;
