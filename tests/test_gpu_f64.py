@@ -175,6 +175,23 @@ def test_rfft64_streaming_batches(order):
     torch.cuda.synchronize()
     assert signals.rel_l2(p[idx].cpu().numpy(), np.abs(ref) ** 2).max() <= TOL, order
     assert not torch.isnan(p).any()
+    # backward over the same batch from random bins (AoS and SoA): the
+    # streaming C2R kernel up to real order 12; imaginary parts of DC and
+    # Nyquist ignored
+    Xb = torch.randn((batch, nb), dtype=torch.complex128, device="cuda", generator=g)
+    refb = np.stack([oracle.rfft_backward(b, n, "f64") for b in Xb[idx].cpu().numpy()])
+    outs = []
+    for lin in (zf.AOS, zf.SOA):
+        o = torch.full((batch, n), np.nan, dtype=torch.float64, device="cuda")
+        plan.backward(Xb if lin == zf.AOS else (Xb.real.contiguous(), Xb.imag.contiguous()), o)
+        torch.cuda.synchronize()
+        e = signals.rel_l2(o[idx].cpu().numpy(), refb).max()
+        print(f"f64 rfft stream backward order={order} batch={batch} lin={lin} relL2={e:.2e}", flush=True)
+        assert e <= TOL, (order, lin)
+        assert not torch.isnan(o).any()
+        outs.append(o)
+    if order >= 8:  # both layouts stream (or both take the one-launch kernel): the same bits
+        assert torch.equal(outs[0], outs[1]), order
 
 
 def test_f64_host_pointers_and_dtype_checks():

@@ -79,8 +79,8 @@ void ZL_CAT(zl_fill_order64_, ZL_LOGN)(zl::OrderTable& tab) {
     tab[R2C][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, double2, 0, GS>();
     tab[R2C][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA, double2, 0, GS>();
     tab[R2C_SQR][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS, double2, 0, GS>();
-    tab[C2R][AOS][AOS] = make_lds_entry<G, C2R, AOS, AOS>();
-    tab[C2R][SOA][AOS] = make_lds_entry<G, C2R, SOA, AOS>();
+    tab[C2R][AOS][AOS] = make_lds_entry<G, C2R, AOS, AOS, double2, 0, GS>();
+    tab[C2R][SOA][AOS] = make_lds_entry<G, C2R, SOA, AOS, double2, 0, GS>();
     tab[R2C_FRAMED][AOS][AOS] = make_lds_entry<G, R2C, AOS, AOS, double2, 1>();
     tab[R2C_FRAMED][AOS][SOA] = make_lds_entry<G, R2C, AOS, SOA, double2, 1>();
     tab[R2C_SQR_FRAMED][AOS][AOS] = make_lds_entry<G, R2C_SQR, AOS, AOS, double2, 1>();

@@ -118,7 +118,7 @@ KernelEntry make_lds_entry() {
     }
     e.tw_total = G::tw_total();
     if constexpr (FRAMED && sizeof(CX) == 8) add_stream_variant<G, KIND, LIN, LOUT, FRAMED>(e);
-    if constexpr (!FRAMED && sizeof(CX) == 16 && KIND != C2R) add_stream_variant<GS, KIND, LIN, LOUT, 0, CX>(e);
+    if constexpr (!FRAMED && sizeof(CX) == 16) add_stream_variant<GS, KIND, LIN, LOUT, 0, CX>(e);
     return e;
 }
 
