@@ -67,27 +67,59 @@ def test_large_soa_config(order, batch):
     assert e <= TOL and e_all <= TOL and rt <= TOL
 
 
-@pytest.mark.parametrize("order", [16, 17, 19])
+@pytest.mark.parametrize("order", [16, 17, 19, 20, 21, 22])
 def test_rfft_large_orders(order):
-    n, nb = 1 << order, (1 << order) // 2 + 1
-    x = signals.uniform(5000 + order, n).astype(np.float32).astype(np.float64)
+    """RFFT forward (bins, both layouts; squared magnitudes) and backward
+    (bins from both layouts, random imaginary parts at DC and Nyquist) on 3
+    frames: the post-twiddle runs in the last HBM pass and the pre-twiddle in
+    the first, which pair column i with column cols - i, and every frame but
+    the first starts off a cache line.  A guard past the last frame stays
+    untouched."""
+    n, nb, batch, guard = 1 << order, (1 << order) // 2 + 1, 3, 64
+    x = signals.uniform(5000 + order, batch * n).reshape(batch, n).astype(np.float32).astype(np.float64)
     plan = zf.RFFT(order)
     xd = torch.from_numpy(x.astype(np.float32)).cuda()
-    ref = oracle.rfft_forward(x, "f64")
+    ref = np.stack([oracle.rfft_forward(r, "f64") for r in x])
+    first = None
     for lay in (zf.AOS, zf.SOA):
-        y = make_out((nb,), lay)
-        plan.forward(xd, y)
+        y = make_out((batch * nb + guard,), lay)
+        yv = y[:batch * nb] if lay == zf.AOS else (y[0][:batch * nb], y[1][:batch * nb])
+        plan.forward(xd, yv)
         Y = fetch(y)
-        assert rel(Y, ref) <= TOL and Y[0].imag == 0.0 and Y[-1].imag == 0.0
-    sq = torch.empty((nb,), dtype=torch.float32, device="cuda")
-    plan.forward_sqr_mag(xd, sq)
+        assert np.all(np.isnan(Y[batch * nb:].real)), "write past the last frame"
+        Y = Y[:batch * nb].reshape(batch, nb)
+        e = rel(Y, ref).max()
+        print(f"rfft order={order} layout={lay} bins relL2={e:.3e}", flush=True)
+        assert e <= TOL
+        assert np.all(Y[:, 0].imag == 0.0) and np.all(Y[:, -1].imag == 0.0)
+        if first is None:
+            first = Y
+        else:
+            assert np.array_equal(Y, first)
+    sq = torch.full((batch * nb + guard,), np.nan, dtype=torch.float32, device="cuda")
+    plan.forward_sqr_mag(xd, sq[:batch * nb])
     torch.cuda.synchronize()
     s = sq.cpu().numpy().astype(np.float64)
-    assert np.linalg.norm(s - np.abs(ref) ** 2) / np.linalg.norm(np.abs(ref) ** 2) <= TOL
-    bins = signals.complex_uniform(6000 + order, 1, nb)[0].astype(np.complex64).astype(np.complex128)
-    want = oracle.rfft_backward(bins, n, "f64")
+    assert np.all(np.isnan(s[batch * nb:]))
+    s = s[:batch * nb].reshape(batch, nb)
+    p = np.abs(ref) ** 2
+    assert (np.linalg.norm(s - p, axis=1) / np.linalg.norm(p, axis=1)).max() <= TOL
+    bins = signals.complex_uniform(6000 + order, batch, nb).astype(np.complex64).astype(np.complex128)
+    want = np.stack([oracle.rfft_backward(b, n, "f64") for b in bins])
+    got = None
     for lay in (zf.AOS, zf.SOA):
-        r = torch.empty((n,), dtype=torch.float32, device="cuda")
-        plan.backward(make_in(bins[None, :], lay) if lay == zf.SOA else to_dev_complex(bins), r)
+        r = torch.full((batch * n + guard,), np.nan, dtype=torch.float32, device="cuda")
+        src = make_in(bins.reshape(1, -1), lay)
+        src = src[0] if lay == zf.AOS else (src[0][0], src[1][0])
+        plan.backward(src, r[:batch * n])
         torch.cuda.synchronize()
-        assert rel(r.cpu().numpy().astype(np.float64), want) <= TOL
+        rr = r.cpu().numpy().astype(np.float64)
+        assert np.all(np.isnan(rr[batch * n:]))
+        rr = rr[:batch * n].reshape(batch, n)
+        e = rel(rr, want).max()
+        print(f"rfft order={order} layout={lay} backward relL2={e:.3e}", flush=True)
+        assert e <= TOL
+        if got is None:
+            got = rr
+        else:
+            assert np.array_equal(rr, got)

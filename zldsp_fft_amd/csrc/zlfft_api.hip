@@ -361,21 +361,33 @@ int ensure_work(zlfft_plan p, size_t bytes) {
 // Multi-pass CFFT of order p->corder on `batch` transforms: pass 0 reads the
 // user layout lin, the last pass writes lout, intermediates ping-pong in work.
 // nt_io: bit 0 when the input, bit 1 when the output is the caller's buffer
-// (touched once: non-temporal access) rather than an internal one
+// (touched once: non-temporal access) rather than an internal one.
+// rk: C2C, or the real kind whose pre-twiddle (C2R: the first pass reads the
+// caller's bins) or post-twiddle (R2C / R2C_SQR: the last pass writes bins or
+// squared magnitudes, db / db_floor as for the STFT) the passes include.
 int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void* in1, int lout, void* out0,
-               void* out1, size_t batch, void* stream, void* work, int nt_io) {
+               void* out1, size_t batch, void* stream, void* work, int nt_io, int rk = zl::C2C, int db = 0,
+               float db_floor = 0.f) {
     const long long n = 1ll << p->corder;
     void* tmp[2] = {work, static_cast<char*>(work) + (size_t)batch * (size_t)n * 2 * p->ss};
     const void* s0 = in0;
     const void* s1 = in1;
     for (int q = 0; q < p->mp; ++q) {
-        const int role = q == 0 ? 0 : (q == p->mp - 1 ? 1 : 2);
-        const int lay = role == 0 ? ((nt_io & 1) ? lin : zl::kInternal)
-                                  : (role == 1 ? ((nt_io & 2) ? lout : zl::kInternal) : zl::AOS);
+        const bool last = q == p->mp - 1;  // writes the caller's output
+        int role = q == 0 ? 0 : (last ? 1 : 2);
+        int lay = role == 0 ? ((nt_io & 1) ? lin : zl::kInternal)
+                            : (role == 1 ? ((nt_io & 2) ? lout : zl::kInternal) : zl::AOS);
+        if (role == 0 && rk == zl::C2R) {
+            role = zl::kRoleC2R;
+            lay = lin;
+        } else if (role == 1 && (rk == zl::R2C || rk == zl::R2C_SQR)) {
+            role = rk == zl::R2C ? zl::kRoleR2C : zl::kRoleSqr;
+            lay = rk == zl::R2C ? lout : zl::AOS;
+        }
         const zl::MacroEntry& e = (p->f64 ? g_macro64 : g_macro)[p->logl[q]][role][lay];
         if (!e.fn) return fail(ZLFFT_ERR_UNSUPPORTED, "no HBM-pass kernel for this layout");
-        void* d0 = role == 1 ? out0 : tmp[q & 1];
-        void* d1 = role == 1 ? out1 : nullptr;
+        void* d0 = last ? out0 : tmp[q & 1];
+        void* d1 = last ? out1 : nullptr;
         zl::MArgs a{};
         a.in0 = s0;
         a.in1 = s1;
@@ -389,6 +401,9 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
         a.logw = p->logw[q];
         a.logh = p->logh[q];
         a.backward = backward;
+        a.rtw = p->d_rtw;
+        a.db = db;
+        a.db_floor = db_floor;
         const size_t grid = batch * (size_t)(n >> p->logl[q]) / (size_t)e.cols;
         if (grid > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
         void* params[] = {&a};
@@ -403,48 +418,7 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
 size_t macro_work_bytes(zlfft_plan p, size_t batch) {
     const size_t n = (size_t)1 << p->corder;
     const size_t bufs = p->mp > 2 ? 2 : 1;
-    size_t b = bufs * batch * n * 2 * p->ss;
-    if (p->kind == ZLFFT_KIND_RFFT) b += batch * n * 2 * p->ss;  // Z
-    return b;
-}
-
-// RFFT post-twiddle / squared magnitude and pre-twiddle around the multi-pass CFFT
-template <class CX>
-int rfft_post(zlfft_plan p, int kind, int lout, const void* z, void* out0, void* out1, size_t batch, void* stream,
-              int db, float db_floor) {
-    const size_t n = (size_t)1 << p->corder;
-    const long long threads = (long long)batch * (long long)(n / 2 + 1);
-    const unsigned grid = (unsigned)((threads + 255) / 256);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    ZL_HIP(plan_launch_check(p));
-    const CX* zz = static_cast<const CX*>(z);
-    const CX* rtw = static_cast<const CX*>(p->d_rtw);
-    if (kind == zl::R2C_SQR)
-        zl::rfft_post_kernel<zl::R2C_SQR, zl::AOS, CX><<<grid, 256, 0, s>>>(zz, out0, nullptr, rtw, p->corder,
-                                                                           (long long)batch, db, db_floor);
-    else if (lout == zl::AOS)
-        zl::rfft_post_kernel<zl::R2C, zl::AOS, CX><<<grid, 256, 0, s>>>(zz, out0, out1, rtw, p->corder, (long long)batch);
-    else
-        zl::rfft_post_kernel<zl::R2C, zl::SOA, CX><<<grid, 256, 0, s>>>(zz, out0, out1, rtw, p->corder, (long long)batch);
-    ZL_HIP(hipGetLastError());
-    return ZLFFT_OK;
-}
-
-template <class CX>
-int rfft_pre(zlfft_plan p, int lin, const void* in0, const void* in1, void* z, size_t batch, void* stream) {
-    const size_t n = (size_t)1 << p->corder;
-    const long long threads = (long long)batch * (long long)(n / 2 + 1);
-    const unsigned grid = (unsigned)((threads + 255) / 256);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    ZL_HIP(plan_launch_check(p));
-    CX* zz = static_cast<CX*>(z);
-    const CX* rtw = static_cast<const CX*>(p->d_rtw);
-    if (lin == zl::AOS)
-        zl::rfft_pre_kernel<zl::AOS, CX><<<grid, 256, 0, s>>>(in0, in1, zz, rtw, p->corder, (long long)batch);
-    else
-        zl::rfft_pre_kernel<zl::SOA, CX><<<grid, 256, 0, s>>>(in0, in1, zz, rtw, p->corder, (long long)batch);
-    ZL_HIP(hipGetLastError());
-    return ZLFFT_OK;
+    return bufs * batch * n * 2 * p->ss;
 }
 
 // The multi-pass transform of one call, on a workspace the caller has sized
@@ -454,19 +428,12 @@ int launch_macro_body(zlfft_plan p, int kind, int lin, int lout, const void* in0
                       void* out1, size_t batch, int backward, void* stream, int db, float db_floor, int user_in) {
     void* work = p->d_work;
     if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work, 3);
-    const size_t n = (size_t)1 << p->corder;
-    void* z = static_cast<char*>(work) + (p->mp > 2 ? 2 : 1) * batch * n * 2 * p->ss;
-    if (kind == zl::R2C || kind == zl::R2C_SQR) {
-        if (int rc = macro_cfft(p, 0, zl::AOS, in0, nullptr, zl::AOS, z, nullptr, batch, stream, work, user_in))
-            return rc;
-        return p->f64 ? rfft_post<double2>(p, kind, lout, z, out0, out1, batch, stream, db, db_floor)
-                      : rfft_post<float2>(p, kind, lout, z, out0, out1, batch, stream, db, db_floor);
-    }
-    // C2R
-    if (int rc = p->f64 ? rfft_pre<double2>(p, lin, in0, in1, z, batch, stream)
-                        : rfft_pre<float2>(p, lin, in0, in1, z, batch, stream))
-        return rc;
-    return macro_cfft(p, 1, zl::AOS, z, nullptr, zl::AOS, out0, nullptr, batch, stream, work, 2);
+    // RFFT: N reals as N/2 complex; the post-twiddle runs in the last pass,
+    // the pre-twiddle in the first
+    if (kind == zl::R2C || kind == zl::R2C_SQR)
+        return macro_cfft(p, 0, zl::AOS, in0, nullptr, lout, out0, out1, batch, stream, work, user_in | 2, kind, db,
+                          db_floor);
+    return macro_cfft(p, 1, lin, in0, in1, zl::AOS, out0, nullptr, batch, stream, work, 3, zl::C2R);
 }
 
 // The workspace serves one stream at a time: a call on another stream than

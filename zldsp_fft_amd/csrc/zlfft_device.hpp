@@ -1143,22 +1143,48 @@ struct MArgs {
     int logw;           // log2 width w of this pass (product of earlier radices)
     int logh;           // split of the two-level macro twiddle
     int backward;       // swap re/im at the user-side load (first pass) and store (last pass)
+    const void* rtw;    // real kinds: RFFT twiddles (wc, ws) for k = 1..N/2, at [k - 1]
+    int db;             // R2C_SQR: write 10 log10(max(|X|^2, db_floor)) instead of |X|^2
+    float db_floor;
 };
 
 // One pass of a multi-pass Stockham DIT transform of N = 2^logn points:
 // radix L = GL::N, width w, N/L butterflies i = (j, k), k = i mod w.
 // Butterfly i reads x[i + m N/L] (m < L), multiplies by W_{L w}^{m k}, takes a
 // length-L DFT in LDS and writes y_q to out[j L w + k + q w].  A workgroup owns
-// C consecutive butterflies of one transform ("columns"): lanes run over
-// columns first so every global access covers C consecutive elements.
+// C butterflies of one transform ("columns"): lanes run over columns first so
+// every global access covers consecutive elements.
 // FIRST: reads the user layout LIN (w = 1, no macro twiddle) and writes its
-// contiguous output block through LDS; LAST: writes the user layout LOUT.
-// Intermediate buffers are AoS float2.
+// output columns (L consecutive elements each) through LDS; LAST: writes the
+// user layout LOUT.  Intermediate buffers are AoS float2.
 // NT: the first pass's input / the last pass's output is the caller's buffer,
 // touched once (non-temporal access); 0 for an internal buffer that the next
-// kernel reads back (the RFFT's inner result, the STFT's frames).
-template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1>
+// kernel reads back (the STFT's frames).
+//
+// RK, the real transforms (an N-point CFFT inside a 2N-point RFFT, reference
+// src/common/zldsp_fft_common_execute.hpp:317-552):
+//   R2C / R2C_SQR (last pass): the RFFT post-twiddle (or |X|^2) is applied to
+//     the pass's outputs, which go out as the N + 1 bins of each frame (layout
+//     LOUT; squared magnitudes: one real each);
+//   C2R (first pass): the pass reads the caller's N + 1 bins (layout LIN) and
+//     applies the pre-twiddle before its sub-FFT.
+// Both need element k with element N - k.  With cols = N / L columns, element
+// k = i + m cols (column i, row m) mirrors to N - k = (cols - i) + (L - 1 - m)
+// cols, and column 0's row m to row (L - m) mod L of itself.  A workgroup of
+// these kinds therefore owns C/2 columns a0 .. a0 + C/2 - 1 (lanes c < C/2)
+// and their mirror columns cols - a0 - c' (lanes C/2 + c'); in the first
+// workgroup lane C/2 takes column cols/2, which mirrors onto itself as column
+// 0 does.  Each lane exchanges its column with the mirror lane's through LDS.
+// Per element, the formula is the one the reference's pair loop uses for that
+// element (k < N - k: its "k" output, otherwise its "N - k" output), with the
+// twiddle of min(k, N - k), so every bin is rounded as in the one-kernel path.
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1, int RK = C2C>
 __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
+    static_assert(RK == C2C || (RK == C2R && FIRST && !LAST) || ((RK == R2C || RK == R2C_SQR) && LAST && !FIRST),
+                  "real kinds fuse into the first (C2R) or last (R2C) pass");
+    using T = CxT<CX>;
+    using S = typename T::S;
+    constexpr bool PAIR = RK != C2C;
     const CX* twa = static_cast<const CX*>(a.twa);
     const CX* twb = static_cast<const CX*>(a.twb);
     __shared__ CX lds[C * GL::LDSN];
@@ -1170,7 +1196,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     // 8-10% slower, radix 256 unchanged).
     constexpr bool lds_tw = GL::logn <= 7 && GL::tw_total() > 0;
     __shared__ CX ltw[lds_tw ? GL::tw_total() : 1];
-    constexpr int L = GL::N, TT = GL::T, R = GL::R, WG = C * TT;
+    constexpr int L = GL::N, TT = GL::T, R = GL::R, WG = C * TT, H = C / 2;
     const int tid = threadIdx.x;
     constexpr int NTR = lds_tw ? (GL::tw_total() + WG - 1) / WG : 0;
     CX twr[NTR > 0 ? NTR : 1];
@@ -1186,12 +1212,26 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     const long long cblocks = cols / C;
     const long long bid = xcd_block(blockIdx.x, gridDim.x);
     const long long b = bid / cblocks;
-    const long long i0 = (bid - b * cblocks) * C;
-    const long long i = i0 + c;
+    const long long bi = bid - b * cblocks;
+    // column of LDS slot / lane s
+    auto column = [&](int s) -> long long {
+        if constexpr (PAIR) {
+            if (s < H) return bi * H + s;
+            return bi == 0 && s == H ? cols / 2 : cols - bi * H - (s - H);
+        } else {
+            return bi * C + s;
+        }
+    };
+    const long long i = column(c);
+    // frame bases: the caller's bins hold N + 1 elements per frame
     const long long base = b * n;
+    const long long ibase = (FIRST && RK == C2R) ? b * (n + 1) : base;
     constexpr int lin = FIRST ? LIN : AOS;
     constexpr int lout = LAST ? LOUT : AOS;
     CX* slds = lds + c * GL::LDSN;
+    // the mirror lane's slot: the other half of the workgroup, or itself for
+    // the self-mirrored columns 0 and cols/2
+    const CX* mlds = lds + (PAIR ? (bi == 0 && (c == 0 || c == H) ? c : (c < H ? c + H : c - H)) : c) * GL::LDSN;
 
     // The user's input (first pass) and output (last pass) are touched once:
     // non-temporal, so they do not evict the intermediate buffer, which the
@@ -1203,16 +1243,49 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     // non-temporal stores reach memory as partial lines: measured 1.27x the
     // output bytes in WRITE_SIZE for the radix-1024 SoA pass.  Such rows go
     // through the cache, which merges the halves of a line that neighbouring
-    // column blocks (renumbered onto one XCD by xcd_block) write.
+    // column blocks (renumbered onto one XCD by xcd_block) write.  The bins
+    // of the real kinds start off a line boundary in every frame but the
+    // first and also go through the cache.
     constexpr int row_bytes = C * ((LAST && LOUT == SOA) ? (int)sizeof(CX) / 2 : (int)sizeof(CX));
-    constexpr bool nt_out = LAST && NT && ZL_MACRO_NT && row_bytes >= ZL_MACRO_NT_MIN_ROW;
+    constexpr bool nt_out = LAST && NT && ZL_MACRO_NT && row_bytes >= ZL_MACRO_NT_MIN_ROW && !PAIR;
     CX v[R];
     if constexpr (nt_in) {
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = ld_c_nt<CX>(a.in0, a.in1, lin, base + i + (long long)(t + TT * j) * cols);
+        for (int j = 0; j < R; ++j) v[j] = ld_c_nt<CX>(a.in0, a.in1, lin, ibase + i + (long long)(t + TT * j) * cols);
     } else {
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, base + i + (long long)(t + TT * j) * cols);
+        for (int j = 0; j < R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, ibase + i + (long long)(t + TT * j) * cols);
+    }
+    if constexpr (FIRST && RK == C2R) {
+        // RFFT pre-twiddle (reference src/common/zldsp_fft_common_execute.hpp:398-468):
+        // Z[0] = ((X0 + XN) / 2, (X0 - XN) / 2) from the real parts of bins 0
+        // and N; otherwise, with x1 = X[k'], x2 = X[N - k'], k' = min(k, N - k):
+        //   Z[k']     = (x1.x - si wc - dr ws, x1.y - si ws + dr wc)
+        //   Z[N - k'] = (x2.x + si wc + dr ws, x2.y - si ws + dr wc)
+        // si = x1.y + x2.y, dr = x1.x - x2.x, (wc, ws) = rtw[k' - 1].
+        const S rn = ld_c<CX>(a.in0, a.in1, lin, ibase + n).x;  // bin N (one cache line per frame)
+#pragma unroll
+        for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
+        __syncthreads();  // every column's bins in LDS
+        const CX* rtw = static_cast<const CX*>(a.rtw);
+        const int ni = (int)n, ii = (int)i, ci = (int)cols;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int m = t + TT * j;
+            const int mm = ii == 0 ? ((L - m) & (L - 1)) : (L - 1 - m);
+            const CX own = slds[GL::pad(m)], mir = mlds[GL::pad(mm)];
+            const int k = ii + m * ci;
+            const bool lo = k < ni - k;
+            const int kk = lo ? k : ni - k;
+            const CX w = rtw[kk > 0 ? kk - 1 : 0];
+            const CX x1 = lo ? own : mir, x2 = lo ? mir : own;
+            const S si = x1.y + x2.y, dr = x1.x - x2.x;
+            const S sg = lo ? S(-1) : S(1);
+            const S zr = T::fma(sg * dr, w.y, T::fma(sg * si, w.x, own.x));
+            const S zi = T::fma(dr, w.x, T::fma(-si, w.y, own.y));
+            v[j] = k == 0 ? T::mk((own.x + rn) * S(0.5), (own.x - rn) * S(0.5)) : T::mk(zr, zi);
+        }
+        __syncthreads();  // mirror reads done before the sub-FFT reuses the LDS
     }
     if (FIRST && a.backward) {
 #pragma unroll
@@ -1247,17 +1320,65 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
     }
     if constexpr (FIRST) {
-        // w = 1: out[i L + q]; the workgroup's output block [i0 L, (i0 + C) L) is contiguous
+        // w = 1: column i's outputs are out[i L + q], L consecutive elements
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
         __syncthreads();
-        const long long ob = base + i0 * L;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int e = tid + WG * j;
             const int cc = e / L, q = e - cc * L;
-            st_c(a.out0, a.out1, lout, ob + e, lds[cc * GL::LDSN + GL::pad(q)]);
+            st_c(a.out0, a.out1, lout, base + column(cc) * L + q, lds[cc * GL::LDSN + GL::pad(q)]);
+        }
+    } else if constexpr (PAIR) {
+        // last pass of an RFFT: cols = w, element k = i + q w.  Post-twiddle
+        // (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552):
+        // X[0] = (Z0.x + Z0.y, 0), X[N] = (Z0.x - Z0.y, 0); otherwise, with
+        // z1 = Z[k'], z2 = Z[N - k'], k' = min(k, N - k):
+        //   X[k']     = (z1.x + si wc - dr ws, z1.y - si ws - dr wc)
+        //   X[N - k'] = (z2.x - si wc + dr ws, z2.y - si ws - dr wc)
+        // si = z1.y + z2.y, dr = z1.x - z2.x, (wc, ws) = rtw[k' - 1].
+        __syncthreads();  // the sub-FFT's LDS reads done
+#pragma unroll
+        for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
+        __syncthreads();
+        // frame b's bins, addressed by 32-bit offsets (N < 2^31)
+        const long long fb = b * (n + 1);
+        void* o0 = RK == R2C && LOUT == AOS ? static_cast<void*>(static_cast<CX*>(a.out0) + fb)
+                                            : static_cast<void*>(static_cast<S*>(a.out0) + fb);
+        void* o1 = RK == R2C && LOUT == SOA ? static_cast<void*>(static_cast<S*>(a.out1) + fb) : nullptr;
+        const CX* rtw = static_cast<const CX*>(a.rtw);
+        const int ni = (int)n, ii = (int)i, ci = (int)cols;
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const int q = t + TT * j;
+            const int qq = ii == 0 ? ((L - q) & (L - 1)) : (L - 1 - q);
+            const CX own = slds[GL::pad(q)], mir = mlds[GL::pad(qq)];
+            const int k = ii + q * ci;
+            const bool lo = k < ni - k;
+            const int kk = lo ? k : ni - k;
+            const CX w = rtw[kk > 0 ? kk - 1 : 0];
+            const CX z1 = lo ? own : mir, z2 = lo ? mir : own;
+            const S si = z1.y + z2.y, dr = z1.x - z2.x;
+            const S sg = lo ? S(-1) : S(1);
+            S xr = T::fma(sg * dr, w.y, T::fma(-sg * si, w.x, own.x));
+            S xi = T::fma(-dr, w.x, T::fma(-si, w.y, own.y));
+            if (k == 0) {
+                xr = own.x + own.y;
+                xi = S(0);
+            }
+            if constexpr (RK == R2C) {
+                st_c(o0, o1, lout, k, T::mk(xr, xi));
+                if (k == 0) st_c(o0, o1, lout, ni, T::mk(own.x - own.y, S(0)));
+            } else {
+                S* o = static_cast<S*>(o0);
+                o[k] = power_out(k == 0 ? xr * xr : T::fma(xr, xr, xi * xi), a.db, a.db_floor);
+                if (k == 0) {
+                    const S ny = own.x - own.y;
+                    o[ni] = power_out(ny * ny, a.db, a.db_floor);
+                }
+            }
         }
     } else {
         const long long w = 1ll << a.logw;
@@ -1272,54 +1393,6 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     }
 }
 
-// RFFT post-twiddle / squared magnitude over a CFFT result Z (AoS, M complex
-// per frame) for sizes whose inner CFFT runs as several HBM passes.  One
-// thread per pair (k, M - k), k = 0..M/2; k = 0 writes DC and Nyquist.
-// Same formulas as the LDS kernel's fused epilogue
-// (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552).
-template <int KIND, int LOUT, class CX = float2>
-__global__ void __launch_bounds__(256) rfft_post_kernel(const CX* __restrict__ z, void* out0, void* out1,
-                                                        const CX* __restrict__ rtw, int logm, long long batch,
-                                                        int db = 0, float db_floor = 0.f) {
-    using T = CxT<CX>;
-    using S = typename T::S;
-    const long long m = 1ll << logm;
-    const long long half = m >> 1;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long f = g / (half + 1);
-    const long long k = g - f * (half + 1);
-    if (f >= batch) return;
-    const CX* zf = z + f * m;
-    const long long fb = f * (m + 1);
-    if (k == 0) {
-        const CX z0 = zf[0];
-        const S dc = z0.x + z0.y, ny = z0.x - z0.y;
-        if constexpr (KIND == R2C) {
-            st_c(out0, out1, LOUT, fb, T::mk(dc, S(0)));
-            st_c(out0, out1, LOUT, fb + m, T::mk(ny, S(0)));
-        } else {
-            reinterpret_cast<S*>(out0)[fb] = power_out(dc * dc, db, db_floor);
-            reinterpret_cast<S*>(out0)[fb + m] = power_out(ny * ny, db, db_floor);
-        }
-        return;
-    }
-    const CX z1 = zf[k], z2 = zf[m - k];
-    const CX w = rtw[k - 1];
-    const S si = z1.y + z2.y, dr = z1.x - z2.x;
-    const S xr_m = T::fma(dr, w.y, T::fma(-si, w.x, z2.x));
-    const S xi_m = T::fma(-dr, w.x, T::fma(-si, w.y, z2.y));
-    const S xr_k = T::fma(-dr, w.y, T::fma(si, w.x, z1.x));
-    const S xi_k = T::fma(-dr, w.x, T::fma(-si, w.y, z1.y));
-    if constexpr (KIND == R2C) {
-        if (k != m - k) st_c(out0, out1, LOUT, fb + k, T::mk(xr_k, xi_k));
-        st_c(out0, out1, LOUT, fb + (m - k), T::mk(xr_m, xi_m));
-    } else {
-        S* o = reinterpret_cast<S*>(out0);
-        if (k != m - k) o[fb + k] = power_out(T::fma(xr_k, xr_k, xi_k * xi_k), db, db_floor);
-        o[fb + (m - k)] = power_out(T::fma(xr_m, xr_m, xi_m * xi_m), db, db_floor);
-    }
-}
-
 // STFT front-end for the multi-pass sizes: frames (hop, window) of a signal
 // copied into a contiguous batch, out[f * n + i] = signal[f * hop + i] * window[i].
 template <class S>
@@ -1330,42 +1403,6 @@ __global__ void __launch_bounds__(256) frame_kernel(const S* __restrict__ signal
     const long long f = g / n, i = g - f * n;
     const S x = signal[f * hop + i];
     out[g] = window ? x * window[i] : x;
-}
-
-// RFFT pre-twiddle for the multi-pass sizes: bins (layout LIN, M+1 per frame)
-// -> Z (AoS, M per frame), the input of the backward CFFT
-// (reference src/common/zldsp_fft_common_execute.hpp:398-468).
-template <int LIN, class CX = float2>
-__global__ void __launch_bounds__(256) rfft_pre_kernel(const void* in0, const void* in1, CX* __restrict__ z,
-                                                       const CX* __restrict__ rtw, int logm, long long batch) {
-    using T = CxT<CX>;
-    using S = typename T::S;
-    const long long m = 1ll << logm;
-    const long long half = m >> 1;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long f = g / (half + 1);
-    const long long k = g - f * (half + 1);
-    if (f >= batch) return;
-    CX* zf = z + f * m;
-    const long long fb = f * (m + 1);
-    if (k == 0) {
-        const S r0 = ld_c<CX>(in0, in1, LIN, fb).x;
-        const S rm = ld_c<CX>(in0, in1, LIN, fb + m).x;
-        zf[0] = T::mk((r0 + rm) * S(0.5), (r0 - rm) * S(0.5));
-        return;
-    }
-    const CX x1 = ld_c<CX>(in0, in1, LIN, fb + k);
-    const CX x2 = ld_c<CX>(in0, in1, LIN, fb + (m - k));
-    const CX w = rtw[k - 1];
-    const S si = x1.y + x2.y, dr = x1.x - x2.x;
-    if (k != m - k) {
-        const S zr_k = T::fma(-dr, w.y, T::fma(-si, w.x, x1.x));
-        const S zi_k = T::fma(dr, w.x, T::fma(-si, w.y, x1.y));
-        zf[k] = T::mk(zr_k, zi_k);
-    }
-    const S zr_m = T::fma(dr, w.y, T::fma(si, w.x, x2.x));
-    const S zi_m = T::fma(dr, w.x, T::fma(-si, w.y, x2.y));
-    zf[m - k] = T::mk(zr_m, zi_m);
 }
 
 }  // namespace zl

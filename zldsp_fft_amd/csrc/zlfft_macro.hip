@@ -23,6 +23,11 @@ void ZL_CAT(zl_fill_macro_, ZL_LOGL)(zl::MacroTable& tab) {
     tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA>();
     tab[1][kInternal] = make_macro_entry<GL, C, 0, 1, AOS, AOS, float2, 0>();
     tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS>();
+    tab[kRoleR2C][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, float2, 1, R2C>();
+    tab[kRoleR2C][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA, float2, 1, R2C>();
+    tab[kRoleSqr][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, float2, 1, R2C_SQR>();
+    tab[kRoleC2R][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS, float2, 1, C2R>();
+    tab[kRoleC2R][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS, float2, 1, C2R>();
 }
 
 void ZL_CAT(zl_fill_macro64_, ZL_LOGL)(zl::MacroTable& tab) {
@@ -36,4 +41,9 @@ void ZL_CAT(zl_fill_macro64_, ZL_LOGL)(zl::MacroTable& tab) {
     tab[1][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA, double2>();
     tab[1][kInternal] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2, 0>();
     tab[2][AOS] = make_macro_entry<GL, C, 0, 0, AOS, AOS, double2>();
+    tab[kRoleR2C][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2, 1, R2C>();
+    tab[kRoleR2C][SOA] = make_macro_entry<GL, C, 0, 1, AOS, SOA, double2, 1, R2C>();
+    tab[kRoleSqr][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2, 1, R2C_SQR>();
+    tab[kRoleC2R][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS, double2, 1, C2R>();
+    tab[kRoleC2R][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS, double2, 1, C2R>();
 }

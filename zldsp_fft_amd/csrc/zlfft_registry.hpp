@@ -185,15 +185,19 @@ struct MacroEntry {
 // (layout = output), 2 = middle pass (layout unused, AoS).  Layout index
 // kInternal (first / last pass only) is AoS on an internal buffer, cached
 // for the kernel that reads it next; AOS / SOA are the caller's buffers.
+// The real kinds' fused passes: role 3 = last pass of an RFFT forward to bins
+// (layout = bins), 4 = last pass to squared magnitudes (AOS only), 5 = first
+// pass of an RFFT backward from bins (layout = bins).
 constexpr int kInternal = 2;
-using MacroTable = MacroEntry[3][3];
+constexpr int kRoleR2C = 3, kRoleSqr = 4, kRoleC2R = 5;
+using MacroTable = MacroEntry[6][3];
 
-template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1>
+template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1, int RK = C2C>
 MacroEntry make_macro_entry() {
     static_assert((long long)sizeof(CX) * (C * GL::LDSN + (GL::logn <= 7 ? GL::tw_total() : 0)) <= 160 * 1024,
                   "HBM pass exceeds the LDS");
     MacroEntry e{};
-    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT, CX, NT>;
+    e.fn = fft_macro_pass<GL, C, FIRST, LAST, LIN, LOUT, CX, NT, RK>;
     e.wg = C * GL::T;
     e.cols = C;
     e.logl = GL::logn;
