@@ -462,6 +462,62 @@ def large_lines(args) -> dict:
         plan.close()
         del ins, outs, xa, ya
         torch.cuda.empty_cache()
+    out.update(large_rfft_lines(torch, lib, stream))
+    return out
+
+
+def large_rfft_lines(torch, lib, stream) -> dict:
+    """Large RFFTs, 8,388,608 reals per call: orders 17 x 64 (inner CFFT of 2
+    HBM passes) and 23 x 1 (3 passes), forward to AoS bins, squared
+    magnitudes and backward, through the C-ABI, with rocFFT's rfft
+    (torch.fft) on the same shapes.  The post-twiddle runs in the last HBM
+    pass and the pre-twiddle in the first."""
+    import oracle
+    import zldsp_fft_amd as zf
+    dev = torch.device("cuda:0")
+    sp = stream.cuda_stream
+    out = {}
+    for order, batch in ((17, 64), (23, 1)):
+        rn, nb = 1 << order, (1 << order) // 2 + 1
+        plan = zf.RFFT(order)
+        h = plan._h
+        g = torch.Generator(device=dev).manual_seed(5)
+        xs = [torch.randn(batch * rn, device=dev, generator=g) for _ in range(ROTATE)]
+        ys = [torch.empty(batch * nb, dtype=torch.complex64, device=dev) for _ in range(ROTATE)]
+        ps = [torch.empty(batch * nb, device=dev) for _ in range(ROTATE)]
+        rs = [torch.empty(batch * rn, device=dev) for _ in range(ROTATE)]
+        flops = 2.5 * rn * order * batch
+        passes = hbm_passes(order - 1)
+        fwd = [lambda i=i: _call(lib, lib.zlfft_rfft_forward_exec, h, xs[i].data_ptr(), zf.AOS, ys[i].data_ptr(),
+                                 None, batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fwd)
+        ref = oracle.rfft_forward(xs[0][:rn].cpu().numpy().astype(np.float64), "f64")
+        got = ys[0][:nb].cpu().numpy().astype(np.complex128)
+        key = f"rfft_o{order}_b{batch}"
+        out[key + "_bins"] = _line((4 * rn + 8 * nb) * batch, flops, batch, wall,
+                                   float(np.linalg.norm(got - ref) / np.linalg.norm(ref)), kern)
+        sq = [lambda i=i: _call(lib, lib.zlfft_rfft_forward_sqr_mag_exec, h, xs[i].data_ptr(), ps[i].data_ptr(),
+                                batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, sq)
+        out[key + "_sqr_mag"] = _line((4 * rn + 4 * nb) * batch, flops, batch, wall, None, kern)
+        for f in fwd:
+            f()
+        bw = [lambda i=i: _call(lib, lib.zlfft_rfft_backward_exec, h, zf.AOS, ys[i].data_ptr(), None,
+                                rs[i].data_ptr(), batch, sp) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, bw)
+        x0 = xs[0][:rn].cpu().numpy().astype(np.float64)
+        r0 = rs[0][:rn].cpu().numpy().astype(np.float64)
+        out[key + "_backward"] = _line((8 * nb + 4 * rn) * batch, flops, batch, wall,
+                                       float(np.linalg.norm(r0 - (rn / 2) * x0) / np.linalg.norm((rn / 2) * x0)), kern)
+        for k in ("_bins", "_sqr_mag", "_backward"):
+            out[key + k]["hbm_passes"] = passes
+        xv = [t.view(batch, rn) for t in xs]
+        fns = [lambda i=i: torch.fft.rfft(xv[i], dim=-1) for i in range(ROTATE)]
+        wall, kern = measure_line(torch, stream, fns)
+        out[f"rocfft_rfft_o{order}_b{batch}"] = _line((4 * rn + 8 * nb) * batch, flops, batch, wall, None, kern)
+        plan.close()
+        del xs, ys, ps, rs, xv
+        torch.cuda.empty_cache()
     return out
 
 

@@ -1127,6 +1127,17 @@ fft_stream_kernel(KArgs a) {
 #ifndef ZL_MACRO_NT
 #define ZL_MACRO_NT 1
 #endif
+// RFFT twiddles of the fused real passes loaded with the data (1) or after
+// the mirror exchange (0).  Measured on 2^22-2^23-point batches, orders 16-22:
+// the first pass of a backward transform 1-6% faster loaded early; the last
+// pass of a forward transform 4-9% slower loaded early (or before its sub-FFT),
+// where the registers held across the sub-FFT cost occupancy.
+#ifndef ZL_MACRO_RTW_EARLY_C2R
+#define ZL_MACRO_RTW_EARLY_C2R 1
+#endif
+#ifndef ZL_MACRO_RTW_EARLY_R2C
+#define ZL_MACRO_RTW_EARLY_R2C 0
+#endif
 
 // Arguments of one HBM pass of a large transform.
 struct MArgs {
@@ -1245,7 +1256,8 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     // through the cache, which merges the halves of a line that neighbouring
     // column blocks (renumbered onto one XCD by xcd_block) write.  The bins
     // of the real kinds start off a line boundary in every frame but the
-    // first and also go through the cache.
+    // first and also go through the cache (measured: non-temporal bins 4-11%
+    // slower).
     constexpr int row_bytes = C * ((LAST && LOUT == SOA) ? (int)sizeof(CX) / 2 : (int)sizeof(CX));
     constexpr bool nt_out = LAST && NT && ZL_MACRO_NT && row_bytes >= ZL_MACRO_NT_MIN_ROW && !PAIR;
     CX v[R];
@@ -1255,6 +1267,21 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     } else {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, ibase + i + (long long)(t + TT * j) * cols);
+    }
+    // real kinds: the RFFT twiddle of each of this lane's elements k,
+    // rtw[min(k, N - k) - 1] (32-bit indices: N < 2^31)
+    const CX* rtw = static_cast<const CX*>(a.rtw);
+    const int ni = (int)n, ii = (int)i, ci = (int)cols;
+    constexpr bool rtw_early = PAIR && (FIRST ? ZL_MACRO_RTW_EARLY_C2R : ZL_MACRO_RTW_EARLY_R2C);
+    CX rw[rtw_early ? R : 1];
+    auto rtw_of = [&](int j) -> CX {
+        const int k = ii + (t + TT * j) * ci;
+        const int kk = k < ni - k ? k : ni - k;
+        return rtw[kk > 0 ? kk - 1 : 0];
+    };
+    if constexpr (rtw_early) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) rw[j] = rtw_of(j);
     }
     if constexpr (FIRST && RK == C2R) {
         // RFFT pre-twiddle (reference src/common/zldsp_fft_common_execute.hpp:398-468):
@@ -1267,8 +1294,6 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
 #pragma unroll
         for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
         __syncthreads();  // every column's bins in LDS
-        const CX* rtw = static_cast<const CX*>(a.rtw);
-        const int ni = (int)n, ii = (int)i, ci = (int)cols;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int m = t + TT * j;
@@ -1276,8 +1301,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
             const CX own = slds[GL::pad(m)], mir = mlds[GL::pad(mm)];
             const int k = ii + m * ci;
             const bool lo = k < ni - k;
-            const int kk = lo ? k : ni - k;
-            const CX w = rtw[kk > 0 ? kk - 1 : 0];
+            const CX w = rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j);
             const CX x1 = lo ? own : mir, x2 = lo ? mir : own;
             const S si = x1.y + x2.y, dr = x1.x - x2.x;
             const S sg = lo ? S(-1) : S(1);
@@ -1348,8 +1372,6 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         void* o0 = RK == R2C && LOUT == AOS ? static_cast<void*>(static_cast<CX*>(a.out0) + fb)
                                             : static_cast<void*>(static_cast<S*>(a.out0) + fb);
         void* o1 = RK == R2C && LOUT == SOA ? static_cast<void*>(static_cast<S*>(a.out1) + fb) : nullptr;
-        const CX* rtw = static_cast<const CX*>(a.rtw);
-        const int ni = (int)n, ii = (int)i, ci = (int)cols;
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const int q = t + TT * j;
@@ -1357,8 +1379,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
             const CX own = slds[GL::pad(q)], mir = mlds[GL::pad(qq)];
             const int k = ii + q * ci;
             const bool lo = k < ni - k;
-            const int kk = lo ? k : ni - k;
-            const CX w = rtw[kk > 0 ? kk - 1 : 0];
+            const CX w = rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j);
             const CX z1 = lo ? own : mir, z2 = lo ? mir : own;
             const S si = z1.y + z2.y, dr = z1.x - z2.x;
             const S sg = lo ? S(-1) : S(1);
