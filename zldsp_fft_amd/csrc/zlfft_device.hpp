@@ -1358,48 +1358,57 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     } else if constexpr (PAIR) {
         // last pass of an RFFT: cols = w, element k = i + q w.  Post-twiddle
         // (reference src/common/zldsp_fft_common_execute.hpp:317-385, 480-552):
-        // X[0] = (Z0.x + Z0.y, 0), X[N] = (Z0.x - Z0.y, 0); otherwise, with
-        // z1 = Z[k'], z2 = Z[N - k'], k' = min(k, N - k):
-        //   X[k']     = (z1.x + si wc - dr ws, z1.y - si ws - dr wc)
-        //   X[N - k'] = (z2.x - si wc + dr ws, z2.y - si ws - dr wc)
-        // si = z1.y + z2.y, dr = z1.x - z2.x, (wc, ws) = rtw[k' - 1].
+        // X[0] = (Z0.x + Z0.y, 0), X[N] = (Z0.x - Z0.y, 0); otherwise, for
+        // k = 1..N/2, with z1 = Z[k], z2 = Z[N - k]:
+        //   X[k]     = (z1.x + si wc - dr ws, z1.y - si ws - dr wc)
+        //   X[N - k] = (z2.x - si wc + dr ws, z2.y - si ws - dr wc)
+        // si = z1.y + z2.y, dr = z1.x - z2.x, (wc, ws) = rtw[k - 1].
+        // A lane computes both bins of each pair whose first element it holds
+        // (rows q < L/2, k < N/2): Z[k] is its own, Z[N - k] comes from the
+        // mirror lane's rows q >= L/2, the only rows that go through LDS.
         __syncthreads();  // the sub-FFT's LDS reads done
 #pragma unroll
-        for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
+        for (int j = R / 2; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
         __syncthreads();
         // frame b's bins, addressed by 32-bit offsets (N < 2^31)
         const long long fb = b * (n + 1);
         void* o0 = RK == R2C && LOUT == AOS ? static_cast<void*>(static_cast<CX*>(a.out0) + fb)
                                             : static_cast<void*>(static_cast<S*>(a.out0) + fb);
         void* o1 = RK == R2C && LOUT == SOA ? static_cast<void*>(static_cast<S*>(a.out1) + fb) : nullptr;
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const int q = t + TT * j;
-            const int qq = ii == 0 ? ((L - q) & (L - 1)) : (L - 1 - q);
-            const CX own = slds[GL::pad(q)], mir = mlds[GL::pad(qq)];
-            const int k = ii + q * ci;
-            const bool lo = k < ni - k;
-            const CX w = rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j);
-            const CX z1 = lo ? own : mir, z2 = lo ? mir : own;
-            const S si = z1.y + z2.y, dr = z1.x - z2.x;
-            const S sg = lo ? S(-1) : S(1);
-            S xr = T::fma(sg * dr, w.y, T::fma(-sg * si, w.x, own.x));
-            S xi = T::fma(-dr, w.x, T::fma(-si, w.y, own.y));
-            if (k == 0) {
-                xr = own.x + own.y;
-                xi = S(0);
-            }
+        // bin k from its value (re, im); squared magnitudes take re^2 for the
+        // real-valued DC and Nyquist bins
+        auto put = [&](int k, S re, S im, bool real) {
             if constexpr (RK == R2C) {
-                st_c(o0, o1, lout, k, T::mk(xr, xi));
-                if (k == 0) st_c(o0, o1, lout, ni, T::mk(own.x - own.y, S(0)));
+                st_c(o0, o1, lout, k, T::mk(re, im));
             } else {
-                S* o = static_cast<S*>(o0);
-                o[k] = power_out(k == 0 ? xr * xr : T::fma(xr, xr, xi * xi), a.db, a.db_floor);
-                if (k == 0) {
-                    const S ny = own.x - own.y;
-                    o[ni] = power_out(ny * ny, a.db, a.db_floor);
-                }
+                static_cast<S*>(o0)[k] = power_out(real ? re * re : T::fma(re, re, im * im), a.db, a.db_floor);
             }
+        };
+        auto pair = [&](int k, CX z1, CX z2, CX w) {
+            const S si = z1.y + z2.y, dr = z1.x - z2.x;
+            put(k, T::fma(-dr, w.y, T::fma(si, w.x, z1.x)), T::fma(-dr, w.x, T::fma(-si, w.y, z1.y)), false);
+            put(ni - k, T::fma(dr, w.y, T::fma(-si, w.x, z2.x)), T::fma(-dr, w.x, T::fma(-si, w.y, z2.y)), false);
+        };
+#pragma unroll
+        for (int j = 0; j < R / 2; ++j) {
+            const int q = t + TT * j;
+            const int k = ii + q * ci;
+            const CX z1 = v[j];
+            if (k == 0) {
+                put(0, z1.x + z1.y, S(0), true);
+                put(ni, z1.x - z1.y, S(0), true);
+            } else {
+                const CX z2 = mlds[GL::pad(ii == 0 ? L - q : L - 1 - q)];
+                pair(k, z1, z2, rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j));
+            }
+        }
+        // column 0's row L/2 (lane t = 0, j = R/2) is bin N/2, its own mirror;
+        // its value is the pair formula's second bin, as in the reference
+        if (ii == 0 && t == 0) {
+            const CX z = v[R / 2];
+            const CX w = rtw[ni / 2 - 1];
+            const S si = z.y + z.y, dr = z.x - z.x;
+            put(ni / 2, T::fma(dr, w.y, T::fma(-si, w.x, z.x)), T::fma(-dr, w.x, T::fma(-si, w.y, z.y)), false);
         }
     } else {
         const long long w = 1ll << a.logw;
