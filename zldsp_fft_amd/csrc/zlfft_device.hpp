@@ -1242,7 +1242,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     CX* slds = lds + c * GL::LDSN;
     // the mirror lane's slot: the other half of the workgroup, or itself for
     // the self-mirrored columns 0 and cols/2
-    const CX* mlds = lds + (PAIR ? (bi == 0 && (c == 0 || c == H) ? c : (c < H ? c + H : c - H)) : c) * GL::LDSN;
+    CX* mlds = lds + (PAIR ? (bi == 0 && (c == 0 || c == H) ? c : (c < H ? c + H : c - H)) : c) * GL::LDSN;
 
     // The user's input (first pass) and output (last pass) are touched once:
     // non-temporal, so they do not evict the intermediate buffer, which the
@@ -1273,43 +1273,60 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     const CX* rtw = static_cast<const CX*>(a.rtw);
     const int ni = (int)n, ii = (int)i, ci = (int)cols;
     constexpr bool rtw_early = PAIR && (FIRST ? ZL_MACRO_RTW_EARLY_C2R : ZL_MACRO_RTW_EARLY_R2C);
-    CX rw[rtw_early ? R : 1];
+    // Both real passes work on the pairs (k, N - k) whose first element a lane
+    // holds: rows q < L/2 (j < R/2), k < N/2, twiddle rtw[k - 1]
+    CX rw[rtw_early ? R / 2 : 1];
     auto rtw_of = [&](int j) -> CX {
         const int k = ii + (t + TT * j) * ci;
-        const int kk = k < ni - k ? k : ni - k;
-        return rtw[kk > 0 ? kk - 1 : 0];
+        return rtw[k > 0 ? k - 1 : 0];
     };
     if constexpr (rtw_early) {
 #pragma unroll
-        for (int j = 0; j < R; ++j) rw[j] = rtw_of(j);
+        for (int j = 0; j < R / 2; ++j) rw[j] = rtw_of(j);
     }
     if constexpr (FIRST && RK == C2R) {
         // RFFT pre-twiddle (reference src/common/zldsp_fft_common_execute.hpp:398-468):
         // Z[0] = ((X0 + XN) / 2, (X0 - XN) / 2) from the real parts of bins 0
-        // and N; otherwise, with x1 = X[k'], x2 = X[N - k'], k' = min(k, N - k):
-        //   Z[k']     = (x1.x - si wc - dr ws, x1.y - si ws + dr wc)
-        //   Z[N - k'] = (x2.x + si wc + dr ws, x2.y - si ws + dr wc)
-        // si = x1.y + x2.y, dr = x1.x - x2.x, (wc, ws) = rtw[k' - 1].
+        // and N; otherwise, for k = 1..N/2, with x1 = X[k], x2 = X[N - k]:
+        //   Z[k]     = (x1.x - si wc - dr ws, x1.y - si ws + dr wc)
+        //   Z[N - k] = (x2.x + si wc + dr ws, x2.y - si ws + dr wc)
+        // si = x1.y + x2.y, dr = x1.x - x2.x, (wc, ws) = rtw[k - 1].
+        // A lane computes both values of each pair whose first element it
+        // holds (rows q < L/2): X[N - k] is read from the mirror lane's rows
+        // q >= L/2 in LDS, and Z[N - k] is written back in its place.
         const S rn = ld_c<CX>(a.in0, a.in1, lin, ibase + n).x;  // bin N (one cache line per frame)
 #pragma unroll
-        for (int j = 0; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
-        __syncthreads();  // every column's bins in LDS
+        for (int j = R / 2; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
+        __syncthreads();  // rows q >= L/2 of every column in LDS
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const int m = t + TT * j;
-            const int mm = ii == 0 ? ((L - m) & (L - 1)) : (L - 1 - m);
-            const CX own = slds[GL::pad(m)], mir = mlds[GL::pad(mm)];
-            const int k = ii + m * ci;
-            const bool lo = k < ni - k;
-            const CX w = rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j);
-            const CX x1 = lo ? own : mir, x2 = lo ? mir : own;
-            const S si = x1.y + x2.y, dr = x1.x - x2.x;
-            const S sg = lo ? S(-1) : S(1);
-            const S zr = T::fma(sg * dr, w.y, T::fma(sg * si, w.x, own.x));
-            const S zi = T::fma(dr, w.x, T::fma(-si, w.y, own.y));
-            v[j] = k == 0 ? T::mk((own.x + rn) * S(0.5), (own.x - rn) * S(0.5)) : T::mk(zr, zi);
+        for (int j = 0; j < R / 2; ++j) {
+            const int q = t + TT * j;
+            const int k = ii + q * ci;
+            const CX x1 = v[j];
+            if (k == 0) {
+                v[j] = T::mk((x1.x + rn) * S(0.5), (x1.x - rn) * S(0.5));
+            } else {
+                CX* pm = mlds + GL::pad(ii == 0 ? L - q : L - 1 - q);
+                const CX x2 = *pm;
+                const CX w = rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j);
+                const S si = x1.y + x2.y, dr = x1.x - x2.x;
+                v[j] = T::mk(T::fma(-dr, w.y, T::fma(-si, w.x, x1.x)), T::fma(dr, w.x, T::fma(-si, w.y, x1.y)));
+                *pm = T::mk(T::fma(dr, w.y, T::fma(si, w.x, x2.x)), T::fma(dr, w.x, T::fma(-si, w.y, x2.y)));
+            }
         }
-        __syncthreads();  // mirror reads done before the sub-FFT reuses the LDS
+        // column 0's row L/2 (lane t = 0, j = R/2) is bin N/2, its own
+        // mirror: the pair formula's second value, as in the reference
+        CX zh = v[R / 2];
+        if (ii == 0 && t == 0) {
+            const CX w = rtw[ni / 2 - 1];
+            const S si = zh.y + zh.y, dr = zh.x - zh.x;
+            zh = T::mk(T::fma(dr, w.y, T::fma(si, w.x, zh.x)), T::fma(dr, w.x, T::fma(-si, w.y, zh.y)));
+        }
+        __syncthreads();  // every pair's Z[N - k] in LDS
+#pragma unroll
+        for (int j = R / 2; j < R; ++j) v[j] = slds[GL::pad(t + TT * j)];
+        if (ii == 0 && t == 0) v[R / 2] = zh;
+        __syncthreads();  // LDS reads done before the sub-FFT reuses the LDS
     }
     if (FIRST && a.backward) {
 #pragma unroll
