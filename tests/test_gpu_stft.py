@@ -91,11 +91,13 @@ def test_stft_defaults_and_errors():
 
 @pytest.mark.parametrize("order,hop,nframes,windowed", [
     (9, 77, 33001, True), (11, 333, 8193, True), (11, 512, 8195, False), (12, 1000, 4099, True),
+    (16, 4097, 21, False), (17, 12345, 9, True), (20, 300001, 3, True),
 ])
 def test_stft_large_batches(order, hop, nframes, windowed):
     """Frame counts above any resident grid, so the persistent streaming form
     runs (including a part-filled last group and odd hops, i.e. frames that
-    start off an 8-byte boundary).  Bins and powers are bit-identical to the
+    start off an 8-byte boundary); orders above 15 run as HBM passes whose
+    first pass reads the frames from the signal.  Bins and powers are bit-identical to the
     plain RFFT of the same frames cut and windowed on the device (one FP32
     product per sample either way); dB levels match 10 log10 of those powers
     to 1e-4 dB."""

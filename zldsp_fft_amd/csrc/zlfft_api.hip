@@ -365,9 +365,11 @@ int ensure_work(zlfft_plan p, size_t bytes) {
 // rk: C2C, or the real kind whose pre-twiddle (C2R: the first pass reads the
 // caller's bins) or post-twiddle (R2C / R2C_SQR: the last pass writes bins or
 // squared magnitudes, db / db_floor as for the STFT) the passes include.
+// lin == FRAMED_IN: the first pass reads frames of the signal in0 (hop,
+// window) for the STFT.
 int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void* in1, int lout, void* out0,
                void* out1, size_t batch, void* stream, void* work, int nt_io, int rk = zl::C2C, int db = 0,
-               float db_floor = 0.f) {
+               float db_floor = 0.f, long long hop = 0, const void* window = nullptr) {
     const long long n = 1ll << p->corder;
     void* tmp[2] = {work, static_cast<char*>(work) + (size_t)batch * (size_t)n * 2 * p->ss};
     const void* s0 = in0;
@@ -377,7 +379,10 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
         int role = q == 0 ? 0 : (last ? 1 : 2);
         int lay = role == 0 ? ((nt_io & 1) ? lin : zl::kInternal)
                             : (role == 1 ? ((nt_io & 2) ? lout : zl::kInternal) : zl::AOS);
-        if (role == 0 && rk == zl::C2R) {
+        if (role == 0 && lin == zl::FRAMED_IN) {
+            role = zl::kRoleFramed;
+            lay = zl::AOS;
+        } else if (role == 0 && rk == zl::C2R) {
             role = zl::kRoleC2R;
             lay = lin;
         } else if (role == 1 && (rk == zl::R2C || rk == zl::R2C_SQR)) {
@@ -404,6 +409,8 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
         a.rtw = p->d_rtw;
         a.db = db;
         a.db_floor = db_floor;
+        a.hop = hop;
+        a.window = window;
         const size_t grid = batch * (size_t)(n >> p->logl[q]) / (size_t)e.cols;
         if (grid > 0x7fffffffull) return fail(ZLFFT_ERR_INVALID_ARG, "batch too large for one launch");
         void* params[] = {&a};
@@ -422,17 +429,18 @@ size_t macro_work_bytes(zlfft_plan p, size_t batch) {
 }
 
 // The multi-pass transform of one call, on a workspace the caller has sized
-// (ensure_work) and ordered for `stream` (work_acquire).  user_in: in0 / in1
-// is the caller's input (0: frames the STFT just wrote).
+// (ensure_work) and ordered for `stream` (work_acquire).  lin == FRAMED_IN
+// (forward real kinds): in0 is the STFT's signal, framed by hop and window.
 int launch_macro_body(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0,
-                      void* out1, size_t batch, int backward, void* stream, int db, float db_floor, int user_in) {
+                      void* out1, size_t batch, int backward, void* stream, int db, float db_floor,
+                      long long hop = 0, const void* window = nullptr) {
     void* work = p->d_work;
     if (kind == zl::C2C) return macro_cfft(p, backward, lin, in0, in1, lout, out0, out1, batch, stream, work, 3);
     // RFFT: N reals as N/2 complex; the post-twiddle runs in the last pass,
     // the pre-twiddle in the first
     if (kind == zl::R2C || kind == zl::R2C_SQR)
-        return macro_cfft(p, 0, zl::AOS, in0, nullptr, lout, out0, out1, batch, stream, work, user_in | 2, kind, db,
-                          db_floor);
+        return macro_cfft(p, 0, lin == zl::FRAMED_IN ? lin : zl::AOS, in0, nullptr, lout, out0, out1, batch, stream,
+                          work, 3, kind, db, db_floor, hop, window);
     return macro_cfft(p, 1, lin, in0, in1, zl::AOS, out0, nullptr, batch, stream, work, 3, zl::C2R);
 }
 
@@ -456,12 +464,13 @@ int work_release(zlfft_plan p, void* stream) {
 }
 
 int launch_macro(zlfft_plan p, int kind, int lin, int lout, const void* in0, const void* in1, void* out0, void* out1,
-                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f, int user_in = 1) {
+                 size_t batch, int backward, void* stream, int db = 0, float db_floor = 0.f, long long hop = 0,
+                 const void* window = nullptr) {
     if (batch == 0) return ZLFFT_OK;
     if (int rc = ensure_work(p, macro_work_bytes(p, batch))) return rc;
     if (int rc = work_acquire(p, stream)) return rc;
     const int rc = launch_macro_body(p, kind, lin, lout, in0, in1, out0, out1, batch, backward, stream, db, db_floor,
-                                     user_in);
+                                     hop, window);
     if (rc) return rc;
     return work_release(p, stream);
 }
@@ -1203,29 +1212,10 @@ int zlfft_rfft_stft_exec(zlfft_plan p, const void* signal, size_t hop, const voi
                            reinterpret_cast<hipStream_t>(stream)));
         return ZLFFT_OK;
     }
-    // multi-pass sizes: the windowed frames go to the workspace (after the
-    // multi-pass buffers), then the RFFT runs on them
-    const size_t n = (size_t)1 << p->order;
-    const size_t mw = macro_work_bytes(p, nframes);
-    if (int rc = ensure_work(p, mw + nframes * n * p->ss)) return rc;
-    if (int rc = work_acquire(p, stream)) return rc;
-    void* frames = static_cast<char*>(p->d_work) + mw;
-    const long long total = (long long)(nframes * n);
-    const unsigned grid = (unsigned)((total + 255) / 256);
-    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    ZL_HIP(plan_launch_check(p));
-    if (p->f64)
-        zl::frame_kernel<double><<<grid, 256, 0, s>>>(static_cast<const double*>(signal),
-                                                      static_cast<const double*>(window), static_cast<double*>(frames),
-                                                      (long long)hop, (long long)n, (long long)nframes);
-    else
-        zl::frame_kernel<float><<<grid, 256, 0, s>>>(static_cast<const float*>(signal), static_cast<const float*>(window),
-                                                     static_cast<float*>(frames), (long long)hop, (long long)n,
-                                                     (long long)nframes);
-    ZL_HIP(hipGetLastError());
-    if (int rc = launch_macro_body(p, kind, 0, lout, frames, nullptr, out0, out1, nframes, 0, stream, db, db_floor, 0))
-        return rc;
-    return work_release(p, stream);
+    // multi-pass sizes: the first HBM pass reads the frames (hop, window)
+    // straight from the signal
+    return launch_macro(p, kind, zl::FRAMED_IN, lout, signal, nullptr, out0, out1, nframes, 0, stream, db, db_floor,
+                        (long long)hop, window);
 }
 
 int zlfft_plan_copy_rfft_twiddles(zlfft_plan p, float* host_out, size_t n_floats) {

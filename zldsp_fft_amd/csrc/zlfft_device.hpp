@@ -29,6 +29,8 @@ enum Kind : int { C2C = 0, R2C = 1, R2C_SQR = 2, C2R = 3 };
 // kernel-table slots of the framed (STFT front-end) real-to-complex kernels
 enum FramedKind : int { R2C_FRAMED = 4, R2C_SQR_FRAMED = 5 };
 enum Layout : int { AOS = 0, SOA = 1 };
+// input "layout" of the first HBM pass of an STFT: frames of a real signal
+constexpr int FRAMED_IN = 2;
 
 constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
 
@@ -1157,6 +1159,10 @@ struct MArgs {
     const void* rtw;    // real kinds: RFFT twiddles (wc, ws) for k = 1..N/2, at [k - 1]
     int db;             // R2C_SQR: write 10 log10(max(|X|^2, db_floor)) instead of |X|^2
     float db_floor;
+    // first pass of an STFT (LIN = FRAMED_IN): transform b is the 2N reals
+    // signal[b hop + r] (in0 = signal), times window[r] (null: none)
+    long long hop;
+    const void* window;
 };
 
 // One pass of a multi-pass Stockham DIT transform of N = 2^logn points:
@@ -1261,7 +1267,22 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     constexpr int row_bytes = C * ((LAST && LOUT == SOA) ? (int)sizeof(CX) / 2 : (int)sizeof(CX));
     constexpr bool nt_out = LAST && NT && ZL_MACRO_NT && row_bytes >= ZL_MACRO_NT_MIN_ROW && !PAIR;
     CX v[R];
-    if constexpr (nt_in) {
+    if constexpr (lin == FRAMED_IN) {
+        // element m of frame b: reals 2m and 2m + 1 of the frame, windowed as
+        // the one-launch STFT kernel does; frames overlap, so cached loads
+        const S* sig = static_cast<const S*>(a.in0) + b * a.hop;
+        const S* win = static_cast<const S*>(a.window);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const long long m2 = 2 * (i + (long long)(t + TT * j) * cols);
+            S x0 = sig[m2], x1 = sig[m2 + 1];
+            if (win) {
+                x0 *= win[m2];
+                x1 *= win[m2 + 1];
+            }
+            v[j] = T::mk(x0, x1);
+        }
+    } else if constexpr (nt_in) {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = ld_c_nt<CX>(a.in0, a.in1, lin, ibase + i + (long long)(t + TT * j) * cols);
     } else {
@@ -1438,18 +1459,6 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
             for (int j = 0; j < R; ++j) st_c(a.out0, a.out1, lout, ob + (long long)(t + TT * j) * w, v[j]);
         }
     }
-}
-
-// STFT front-end for the multi-pass sizes: frames (hop, window) of a signal
-// copied into a contiguous batch, out[f * n + i] = signal[f * hop + i] * window[i].
-template <class S>
-__global__ void __launch_bounds__(256) frame_kernel(const S* __restrict__ signal, const S* __restrict__ window,
-                                                    S* __restrict__ out, long long hop, long long n, long long frames) {
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= frames * n) return;
-    const long long f = g / n, i = g - f * n;
-    const S x = signal[f * hop + i];
-    out[g] = window ? x * window[i] : x;
 }
 
 }  // namespace zl

@@ -28,6 +28,7 @@ void ZL_CAT(zl_fill_macro_, ZL_LOGL)(zl::MacroTable& tab) {
     tab[kRoleSqr][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, float2, 1, R2C_SQR>();
     tab[kRoleC2R][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS, float2, 1, C2R>();
     tab[kRoleC2R][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS, float2, 1, C2R>();
+    tab[kRoleFramed][AOS] = make_macro_entry<GL, C, 1, 0, FRAMED_IN, AOS, float2, 0>();
 }
 
 void ZL_CAT(zl_fill_macro64_, ZL_LOGL)(zl::MacroTable& tab) {
@@ -46,4 +47,5 @@ void ZL_CAT(zl_fill_macro64_, ZL_LOGL)(zl::MacroTable& tab) {
     tab[kRoleSqr][AOS] = make_macro_entry<GL, C, 0, 1, AOS, AOS, double2, 1, R2C_SQR>();
     tab[kRoleC2R][AOS] = make_macro_entry<GL, C, 1, 0, AOS, AOS, double2, 1, C2R>();
     tab[kRoleC2R][SOA] = make_macro_entry<GL, C, 1, 0, SOA, AOS, double2, 1, C2R>();
+    tab[kRoleFramed][AOS] = make_macro_entry<GL, C, 1, 0, FRAMED_IN, AOS, double2, 0>();
 }

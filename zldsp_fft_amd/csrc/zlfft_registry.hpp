@@ -187,10 +187,11 @@ struct MacroEntry {
 // for the kernel that reads it next; AOS / SOA are the caller's buffers.
 // The real kinds' fused passes: role 3 = last pass of an RFFT forward to bins
 // (layout = bins), 4 = last pass to squared magnitudes (AOS only), 5 = first
-// pass of an RFFT backward from bins (layout = bins).
+// pass of an RFFT backward from bins (layout = bins), 6 = first pass of an
+// STFT, reading frames of the caller's signal (layout AOS only).
 constexpr int kInternal = 2;
-constexpr int kRoleR2C = 3, kRoleSqr = 4, kRoleC2R = 5;
-using MacroTable = MacroEntry[6][3];
+constexpr int kRoleR2C = 3, kRoleSqr = 4, kRoleC2R = 5, kRoleFramed = 6;
+using MacroTable = MacroEntry[7][3];
 
 template <class GL, int C, int FIRST, int LAST, int LIN, int LOUT, class CX = float2, int NT = 1, int RK = C2C>
 MacroEntry make_macro_entry() {
