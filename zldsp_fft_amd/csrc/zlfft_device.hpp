@@ -1129,16 +1129,15 @@ fft_stream_kernel(KArgs a) {
 #ifndef ZL_MACRO_NT
 #define ZL_MACRO_NT 1
 #endif
-// RFFT twiddles of the fused real passes loaded with the data (1) or after
-// the mirror exchange (0).  Measured on 2^22-2^23-point batches, orders 16-22:
-// the first pass of a backward transform 1-6% faster loaded early; the last
-// pass of a forward transform 4-9% slower loaded early (or before its sub-FFT),
-// where the registers held across the sub-FFT cost occupancy.
+// RFFT twiddles of the fused real passes.  The first pass of a backward
+// transform loads them with the data (1; measured on 2^22-2^23-point batches,
+// orders 16-22: 1-6% faster than after the mirror exchange).  The last pass of
+// a forward transform loads them right after its sub-FFT, ahead of the
+// exchange's barriers: 0.3-6% faster than after the exchange, while loading
+// them with the data or before the sub-FFT was 4-9% slower (registers held
+// across the sub-FFT cost occupancy).
 #ifndef ZL_MACRO_RTW_EARLY_C2R
 #define ZL_MACRO_RTW_EARLY_C2R 1
-#endif
-#ifndef ZL_MACRO_RTW_EARLY_R2C
-#define ZL_MACRO_RTW_EARLY_R2C 0
 #endif
 
 // Arguments of one HBM pass of a large transform.
@@ -1293,7 +1292,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     // rtw[min(k, N - k) - 1] (32-bit indices: N < 2^31)
     const CX* rtw = static_cast<const CX*>(a.rtw);
     const int ni = (int)n, ii = (int)i, ci = (int)cols;
-    constexpr bool rtw_early = PAIR && (FIRST ? ZL_MACRO_RTW_EARLY_C2R : ZL_MACRO_RTW_EARLY_R2C);
+    constexpr bool rtw_early = PAIR && FIRST && ZL_MACRO_RTW_EARLY_C2R;
     // Both real passes work on the pairs (k, N - k) whose first element a lane
     // holds: rows q < L/2 (j < R/2), k < N/2, twiddle rtw[k - 1]
     CX rw[rtw_early ? R / 2 : 1];
@@ -1404,6 +1403,9 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         // A lane computes both bins of each pair whose first element it holds
         // (rows q < L/2, k < N/2): Z[k] is its own, Z[N - k] comes from the
         // mirror lane's rows q >= L/2, the only rows that go through LDS.
+        CX w2[R / 2];  // twiddles of the pairs, in flight during the exchange
+#pragma unroll
+        for (int j = 0; j < R / 2; ++j) w2[j] = rtw_of(j);
         __syncthreads();  // the sub-FFT's LDS reads done
 #pragma unroll
         for (int j = R / 2; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
@@ -1437,7 +1439,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
                 put(ni, z1.x - z1.y, S(0), true);
             } else {
                 const CX z2 = mlds[GL::pad(ii == 0 ? L - q : L - 1 - q)];
-                pair(k, z1, z2, rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j));
+                pair(k, z1, z2, w2[j]);
             }
         }
         // column 0's row L/2 (lane t = 0, j = R/2) is bin N/2, its own mirror;
