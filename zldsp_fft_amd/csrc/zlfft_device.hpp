@@ -1288,13 +1288,12 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
 #pragma unroll
         for (int j = 0; j < R; ++j) v[j] = ld_c<CX>(a.in0, a.in1, lin, ibase + i + (long long)(t + TT * j) * cols);
     }
-    // real kinds: the RFFT twiddle of each of this lane's elements k,
-    // rtw[min(k, N - k) - 1] (32-bit indices: N < 2^31)
+    // real kinds: both passes work on the pairs (k, N - k) whose first
+    // element a lane holds, rows q < L/2 (j < R/2, k < N/2), with the RFFT
+    // twiddle rtw[k - 1] (32-bit indices: N < 2^31)
     const CX* rtw = static_cast<const CX*>(a.rtw);
     const int ni = (int)n, ii = (int)i, ci = (int)cols;
     constexpr bool rtw_early = PAIR && FIRST && ZL_MACRO_RTW_EARLY_C2R;
-    // Both real passes work on the pairs (k, N - k) whose first element a lane
-    // holds: rows q < L/2 (j < R/2), k < N/2, twiddle rtw[k - 1]
     CX rw[rtw_early ? R / 2 : 1];
     auto rtw_of = [&](int j) -> CX {
         const int k = ii + (t + TT * j) * ci;
