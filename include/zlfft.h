@@ -227,7 +227,7 @@ int zlfft_ragged_destroy(zlfft_ragged ragged);
  *   ZLFFT_STFT_POWER: N/2+1 squared magnitudes per frame (as forward_sqr_mag)
  *   ZLFFT_STFT_DB   : 10 log10(max(|X|^2, db_floor)) per bin (db_floor > 0)
  * LDS-resident orders run as one kernel that frames and windows at load time;
- * larger orders gather the windowed frames into the plan workspace first.
+ * larger orders read the windowed frames in their first HBM pass.
  * signal must hold (nframes - 1) * hop + N values (the call cannot check it). */
 #define ZLFFT_STFT_BINS 0
 #define ZLFFT_STFT_POWER 1
