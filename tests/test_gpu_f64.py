@@ -117,11 +117,13 @@ def test_cfft64_multipass_orders(order, batch):
     print(f"f64 cfft order {order}: fwd {e:.2e} round trip {e2:.2e}", flush=True)
 
 
-@pytest.mark.parametrize("order", [1, 2, 3, 5, 8, 11, 14, 15, 16, 17])
+@pytest.mark.parametrize("order", [1, 2, 3, 5, 8, 11, 14, 15, 16, 17, 18, 19, 20, 21])
 def test_rfft64(order):
     """RFFT<double>: forward AoS / SoA, forward_sqr_mag and backward (scaled by
     N/2, imaginary parts of bins 0 and N/2 ignored), against the FP64 oracle;
-    exact zero imaginary parts at DC and Nyquist."""
+    exact zero imaginary parts at DC and Nyquist.  Orders 15-21 run as HBM
+    passes with the post-/pre-twiddle in the last / first pass, whose
+    workgroups hold 32, 16 or 8 column pairs (radix 128-256, 512, 1024)."""
     n = 1 << order
     nb = n // 2 + 1
     batch = 3
