@@ -207,7 +207,7 @@ int zlfft_ragged_create(zlfft_ragged *ragged, const zlfft_ragged_desc *descs, si
 /* As zlfft_ragged_create, with the points per thread (log2: 3, 4 or 5) of each
  * CFFT order 0..14 given in logr[15] instead of the tuned table
  * (zlfft_tune.txt next to the library, or $ZLFFT_TUNE_FILE); for the
- * autotuner.  Accepted: 8 points for CFFT orders 3-7, 16 for 4-13, 32 for
+ * autotuner.  Accepted: 8 points for CFFT orders 3-7, 16 for 4-12, 32 for
  * 8-14 (an RFFT of order n uses CFFT order n - 1); ZLFFT_ERR_INVALID_ARG
  * otherwise. */
 int zlfft_ragged_create_config(zlfft_ragged *ragged, const zlfft_ragged_desc *descs, size_t count,

@@ -33,9 +33,12 @@ LOGRS = (3, 4, 5)      # log2 points per thread the ragged kernel supports
 
 
 def legal(order: int, logr: int) -> bool:
-    """Whether the ragged kernel runs CFFT order `order` at 2^logr points per
-    thread: 8 for orders 3-7, 16 for 4-13, 32 for 8-14."""
-    return (logr == 3 and 3 <= order <= 7) or (logr == 4 and 4 <= order <= 13) or (logr == 5 and 8 <= order <= 14)
+    """Whether the ragged kernels run CFFT order `order` at 2^logr points per
+    thread: 8 for orders 3-7, 16 for 4-12, 32 for 8-14 (a transform's
+    2^(order - logr) threads fit a workgroup: 256 threads up to order 13, 512
+    at order 14)."""
+    ok = (logr == 3 and 3 <= order <= 7) or (logr == 4 and 4 <= order <= 13) or (logr == 5 and 8 <= order <= 14)
+    return ok and (1 << (order - logr)) <= (512 if order >= 14 else 256)
 
 
 def default_logr(order: int) -> int:

@@ -2,12 +2,11 @@
 // layouts in one persistent launch.
 //
 // The host sorts the transforms into work items of one (CFFT order, variant):
-// up to TPW = RWG / T transforms (T = 2^n / R threads each, R = 8, 16 or 32
-// points per thread, chosen per order) that one workgroup of RWG threads
-// handles together.  It deals the
-// items to the workgroups longest first, each to the workgroup with the least
-// work so far (by bytes moved), so each workgroup owns a fixed run of items
-// and no queue is needed.  A workgroup streams through its run the way the
+// up to TPW = WG / T transforms (T = 2^n / R threads each, R = 8, 16 or 32
+// points per thread, chosen per order) that one workgroup of WG threads
+// handles together.  It deals the items to the workgroups longest first, each
+// to the workgroup with the least work so far (by bytes moved), so each
+// workgroup owns a fixed run of items and no queue is needed.  A workgroup streams through its run the way the
 // batched streaming kernel streams through a batch: the input of the next item
 // is loaded into registers while the current one is transformed and stored,
 // and the descriptors two items ahead.  The Stockham and RFFT twiddles come
@@ -19,11 +18,18 @@
 
 namespace zl {
 
-// threads per ragged workgroup: 8 waves, so each thread may hold 256 VGPRs
-// (the input of two items, 32 points each); the LDS holds one workgroup per CU
-constexpr int RWG = 512;
-constexpr int kRaggedMinOrder = 3;  // CFFT orders handled by the ragged kernel
+constexpr int kRaggedMinOrder = 3;  // CFFT orders handled by the ragged kernels
 constexpr int kRaggedMaxOrder = 14;
+// Two kernels.  A batch without order-14 transforms runs in 256-thread
+// workgroups (4 waves of up to 256 VGPRs, LDS tiles of at most 73 KB: two
+// workgroups per CU); a batch with them, whose 128 KiB transforms fill a CU's
+// LDS, in 512-thread workgroups, one per CU.  Measured on 10,000 transforms
+// of orders 6-13: the small kernel at 1.17x the rate of the large one (order
+// 12 alone 1.28x, order 13 1.16x).  Every order but 14 must therefore fit
+// 256 threads per transform: ragged_wg(n).
+constexpr int kRaggedLargeOrder = 14;
+constexpr int kRaggedSmallWG = 256, kRaggedLargeWG = 512;
+constexpr int ragged_wg(int n) { return n >= kRaggedLargeOrder ? kRaggedLargeWG : kRaggedSmallWG; }
 
 // One transform of a ragged batch (device copy of the user descriptor).
 struct RDesc {
@@ -75,31 +81,39 @@ struct RArgs {
     int sink;             // index of the sink descriptor
 };
 
-// Points per thread (log2) the ragged kernel supports for CFFT order n: 8
-// for orders 3-7, 16 for orders 4-13, 32 for orders 8-14 (at most RWG
-// threads per transform; each pair is a pass chain of its own in the kernel,
-// so the small-order / large-point pairs no order is tuned to are left out)
+// Points per thread (log2) the ragged kernels support for CFFT order n: 8
+// for orders 3-7, 16 for orders 4-12, 32 for orders 8-14, with at most a
+// workgroup's threads per transform (each pair is a pass chain of its own in
+// a kernel, so the small-order / large-point pairs no order is tuned to are
+// left out)
 constexpr int kRaggedMinLogr = 3;
 constexpr int kRaggedMaxLogr = 5;
 constexpr bool ragged_logr_ok(int n, int logr) {
-    return (logr == 3 && n >= 3 && n <= 7) || (logr == 4 && n >= 4 && n <= 13) || (logr == 5 && n >= 8 && n <= 14);
+    return ((logr == 3 && n >= 3 && n <= 7) || (logr == 4 && n >= 4 && n <= 13) || (logr == 5 && n >= 8 && n <= 14)) &&
+           n - logr >= 0 && (1 << (n - logr)) <= ragged_wg(n);
 }
 
-// LDS of a ragged workgroup: TPW = RWG / T transforms of 2^n points
-// (T = 2^n / R threads each) with one pad slot per 16, the largest over the
-// orders and point counts handled.
+// LDS of a ragged workgroup of WG threads for orders [MINO, MAXO]:
+// TPW = WG / T transforms of 2^n points (T = 2^n / R threads each) with one
+// pad slot per 16, the largest over the orders and point counts handled.
+template <int WG, int MINO, int MAXO>
 constexpr int ragged_lds_elems() {
     int m = 0;
-    for (int n = kRaggedMinOrder; n <= kRaggedMaxOrder; ++n)
+    for (int n = MINO; n <= MAXO; ++n)
         for (int logr = kRaggedMinLogr; logr <= kRaggedMaxLogr; ++logr) {
             if (!ragged_logr_ok(n, logr)) continue;
-            const int N = 1 << n, T = N >> logr, tpw = T >= RWG ? 1 : RWG / T;
+            const int N = 1 << n, T = N >> logr, tpw = WG / T;
             const int ldsn = N + (N >> 4) + 1;
             if (tpw * ldsn > m) m = tpw * ldsn;
         }
     return m;
 }
-static_assert(8ll * (ragged_lds_elems() + kMtTotal) <= 160 * 1024, "ragged LDS exceeds the CU's");
+static_assert(8ll * (ragged_lds_elems<kRaggedSmallWG, kRaggedMinOrder, kRaggedLargeOrder - 1>() + kMtTotal) <=
+                  80 * 1024,
+              "two small ragged workgroups per CU");
+static_assert(8ll * (ragged_lds_elems<kRaggedLargeWG, kRaggedMinOrder, kRaggedMaxOrder>() + kMtTotal) <=
+                  160 * 1024,
+              "ragged LDS exceeds the CU's");
 
 __device__ __forceinline__ int pad16(int e) { return e + (e >> 4); }
 
@@ -200,18 +214,18 @@ __device__ __forceinline__ void rgc_pass(float2 (&v)[G::R], float2* slds, const 
     }
 }
 
-// the pass chain of order n (one compare per order the kernel runs at 2^LOGR
-// points per thread)
-template <int LOGR, int NL = kRaggedMinOrder>
+// the pass chain of order n (one compare per order in [NL, MAXO] the kernel
+// runs at 2^LOGR points per thread)
+template <int LOGR, int NL, int MAXO>
 __device__ __forceinline__ void rgc_passes(int n, float2 (&v)[1 << LOGR], float2* slds, const float2* mt, int t) {
-    if constexpr (NL <= kRaggedMaxOrder) {
+    if constexpr (NL <= MAXO) {
         if constexpr (ragged_logr_ok(NL, LOGR)) {
             if (n == NL) {
                 rgc_pass<RGeo<NL, LOGR>, 0>(v, slds, mt, t);
                 return;
             }
         }
-        rgc_passes<LOGR, NL + 1>(n, v, slds, mt, t);
+        rgc_passes<LOGR, NL + 1, MAXO>(n, v, slds, mt, t);
     }
 }
 
@@ -267,7 +281,7 @@ __device__ __forceinline__ void rg_prefetch(float2 (&p)[1 << LOGR], float2& px, 
 // after_passes() runs once the passes are done, before the epilogue's stores:
 // loads it issues are ahead of the stores in the memory counter, so waiting
 // for them later does not wait for this item's stores.
-template <int LOGR, class AfterPasses>
+template <int LOGR, int MINO, int MAXO, class AfterPasses>
 __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, const RItem& it, const RDesc& d,
                                              float2* lds, const float2* mt, int tid, AfterPasses after_passes) {
     constexpr int R = 1 << LOGR;
@@ -324,7 +338,7 @@ __device__ __forceinline__ void rg_transform(float2 (&v)[1 << LOGR], float2 px, 
         for (int j = 0; j < R; ++j) v[j] = swapri(v[j]);
     }
 
-    rgc_passes<LOGR>(n, v, slds, mt, t);
+    rgc_passes<LOGR, MINO, MAXO>(n, v, slds, mt, t);
     after_passes();
 
     if (kind == R2C || kind == R2C_SQR) {

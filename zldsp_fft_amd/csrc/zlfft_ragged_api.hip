@@ -157,29 +157,39 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
     std::stable_sort(t.begin(), t.end(), [](const Tmp& a, const Tmp& b) {
         return a.corder != b.corder ? a.corder > b.corder : a.key < b.key;
     });
+    // The kernel: a batch with order-14 transforms runs in 512-thread
+    // workgroups (one per CU, every order), one without in 256-thread
+    // workgroups (two per CU).  Splitting a batch between the two kernels was
+    // measured slower on the 10,000-transform mix of orders 6-14 (141.8
+    // against 129.0 us; as parallel graph branches 151.8): order 14 alone
+    // leaves the large kernel's workgroups unevenly loaded.
+    const int cls = count > 0 && t[0].corder >= kRaggedLargeOrder ? 1 : 0;  // t is sorted by order, largest first
+    const int wg = cls ? kRaggedLargeWG : kRaggedSmallWG;
     std::vector<RDesc> hd(count + 1);
     std::vector<RItem> hi;
     for (size_t i = 0; i < count; ++i) hd[i] = t[i].d;
     for (size_t i = 0; i < count;) {
         const int n = t[i].corder, key = t[i].key;
-        const int T = 1 << (n - lr[n]), tpw = RWG / T;
+        const int T = 1 << (n - lr[n]), tpw = wg / T;
         size_t j = i;
         while (j < count && (int)(j - i) < tpw && t[j].corder == n && t[j].key == key) ++j;
         hi.push_back(RItem{(int)i, (int)(j - i), n, key});
         i = j;
     }
 
-    int dev = 0, cus = 0, per_cu = 0;
+    int dev = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, zl_ragged_kernel(), RWG, 0);
-    if (e != hipSuccess) return hipf(e, "occupancy query");
-    const int grid = std::max(1, std::min(std::max(per_cu, 1) * cus, (int)hi.size()));
+    if (e != hipSuccess) return hipf(e, "device query");
 
     // Deal the items to the workgroups: largest first, each to the workgroup
     // with the least bytes so far (plus a fixed cost per item).  A
     // workgroup's run is ordered by points per thread (one loop each in the
     // kernel), then largest first.
+    int per_cu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, zl_ragged_kernel(cls), wg, 0);
+    if (e != hipSuccess) return hipf(e, "occupancy query");
+    const int grid = std::max(1, std::min(std::max(per_cu, 1) * cus, (int)hi.size()));
     constexpr long long kItemCost = 16384;
     std::vector<long long> cost(hi.size());
     std::vector<int> order(hi.size());
@@ -258,7 +268,7 @@ int zlfft_ragged_create_config(zlfft_ragged* out, const zlfft_ragged_desc* descs
     e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return cleanup(hipf(e, "hipStreamBeginCapture"));
     void* kp[] = {&a};
-    e = hipLaunchKernel(zl_ragged_kernel(), dim3(grid), dim3(RWG), kp, 0, r->stream);
+    e = hipLaunchKernel(zl_ragged_kernel(cls), dim3(grid), dim3(wg), kp, 0, r->stream);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(r->stream, &g);
     r->graph = g;

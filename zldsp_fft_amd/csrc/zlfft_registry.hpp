@@ -223,8 +223,9 @@ int zl_read_inject();
 // per workgroup, or the persistent streaming form), or null
 void* zl_mfma_kernel(int prec, int backward, int streaming);
 
-// ragged-batch kernel (zlfft_ragged_k.hip)
-void* zl_ragged_kernel();
+// ragged-batch kernels (zlfft_ragged_k.hip): orders up to 13 (large = 0) or
+// order 14 (large = 1)
+void* zl_ragged_kernel(int large);
 
 #define ZL_DECLARE_MACRO(n)                            \
     void zl_fill_macro_##n(zl::MacroTable& tab);       \
