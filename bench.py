@@ -312,15 +312,15 @@ def mfma_variants(x, y, stream, lib, args, plain_step) -> dict:
 
 # ---------------------------------------------------------------- secondary lines
 
-def ragged_section(args) -> dict:
-    """The ragged batch: 10,000 transforms with orders uniform in 6..14,
-    CFFT / RFFT, forward / backward, AoS / SoA (50/50 each), in one
-    graph-launched persistent kernel.  Throughput in algorithmic bytes (one
-    read of each input, one write of each output) per graph launch."""
+def _ragged_line(orders) -> dict:
+    """10,000 transforms with orders uniform in `orders`, CFFT / RFFT,
+    forward / backward, AoS / SoA (50/50 each), in one graph-launched
+    persistent kernel.  Throughput in algorithmic bytes (one read of each
+    input, one write of each output) per graph launch."""
     import torch
     import zldsp_fft_amd as zf
     from zldsp_fft_amd import workloads
-    descs = workloads.ragged_mix(10000, 6)
+    descs = workloads.ragged_mix(10000, 6, orders)
     nbytes = sum(workloads.algorithmic_bytes(d) for d in descs)
     rb = zf.RaggedBatch(descs)
     stream = torch.cuda.current_stream()
@@ -334,11 +334,19 @@ def ragged_section(args) -> dict:
     rb.close()
     gbs = nbytes / (wall * 1e-3) / 1e9
     kg = nbytes / (kern * 1e-3) / 1e9
-    return {"workload": "ragged_10000_orders6-14_mixed", "bytes_per_launch": nbytes, "ms_per_launch": round(wall, 5),
-            "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+    return {"workload": f"ragged_10000_orders{orders[0]}-{orders[1]}_mixed", "bytes_per_launch": nbytes,
+            "ms_per_launch": round(wall, 5), "GB/s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
             "kernel_ms_mean": round(kern, 5), "kernel_GB/s": round(kg, 1),
             "transforms_per_s": round(len(descs) / (wall * 1e-3), 1),
             "check": "every output vs the per-transform plain path <= 1e-6 rel-L2 in tests/test_gpu_ragged.py"}
+
+
+def ragged_section(args) -> dict:
+    """The ragged batch of orders 6..14 (512-thread kernel); beside it the
+    same mix without order 14, which runs in the 256-thread kernel."""
+    out = _ragged_line((6, 14))
+    out["without_order_14"] = _ragged_line((6, 13))
+    return out
 
 
 def analyzer_lines(args) -> dict:
@@ -838,9 +846,10 @@ def main():
         torch.cuda.empty_cache()
         out["ragged"] = ragged_section(args)
         if not args.no_copy_ref:
-            c = copy_reference(out["ragged"]["bytes_per_launch"])
-            out["ragged"]["copy_GB/s"] = c["GB/s"]
-            out["ragged"]["of_copy"] = round(out["ragged"]["GB/s"] / c["GB/s"], 4)
+            for d in (out["ragged"], out["ragged"]["without_order_14"]):
+                c = copy_reference(d["bytes_per_launch"])
+                d["copy_GB/s"] = c["GB/s"]
+                d["of_copy"] = round(d["GB/s"] / c["GB/s"], 4)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     if rel > 1e-6:
