@@ -372,7 +372,8 @@ __device__ __forceinline__ CX framed_pair(const KArgs& a, long long frame, long 
 // [kMfmaTwPass2 + 256 m + k] (k < 256); divided by 16 for the FP16 kernels.
 // (Factoring pass 2 as W_256^{m a} W_4096^{m b}, k = 16 a + b, cuts the table
 // to 4 KiB and allows 3 workgroups per CU, but its extra complex product per
-// value measured 1.5% slower.)
+// value measured slower: 1.5% with the earlier FP16 split, 7% with the current
+// one, and 7% at 4 workgroups per CU, capped at 128 VGPRs.)
 constexpr int kMfmaTwPass2 = 256;
 constexpr int kMfmaTwTotal = kMfmaTwPass2 + 16 * 256;
 
