@@ -271,6 +271,8 @@ struct zlfft_plan_s {
     int stream_blocks[6][2][2] = {};  // resident workgroups per CU of each streaming kernel
     int mfma_blocks = 0;              // resident workgroups per CU of the persistent MFMA kernel
     void* d_rtw = nullptr;      // RFFT (wc, ws), complex type of the plan
+    double2* d_rtwf = nullptr;  // FP32 multi-pass RFFT: FP64 twiddle factors and steps (MArgs::rtwf)
+    int rtw_h = 0, rtw_b = 0, rtw_s = 0;
     int n_rtw = 0;
     // multi-pass transforms (corder > kMaxLdsOrder)
     bool macro = false;
@@ -407,6 +409,10 @@ int macro_cfft(zlfft_plan p, int backward, int lin, const void* in0, const void*
         a.logh = p->logh[q];
         a.backward = backward;
         a.rtw = p->d_rtw;
+        a.rtwf = p->d_rtwf;
+        a.rtwh = p->rtw_h;
+        a.rtwb = p->rtw_b;
+        a.rtws = p->rtw_s;
         a.db = db;
         a.db_floor = db_floor;
         a.hop = hop;
@@ -981,6 +987,25 @@ int zlfft_plan_create(zlfft_plan* plan, int kind, int precision, size_t order) {
             e = hipGetLastError();
             if (e != hipSuccess) return cleanup(hip_fail(e, "rfft twiddle generation"));
         }
+        if (p->macro && !f64) {
+            // FP64 W_N^e for the fused passes' twiddles: A[k >> h], B[k & (2^h - 1)]
+            // for k = 0..N/4, then [idx 0, 1] of W^{N/32 idx} and of W^{N/64 idx}
+            // (the steps N/(2 R) of the real size N for R = 16 and 32 points per thread)
+            const int lk = (int)order - 2;  // log2 (N/4)
+            const int h = (lk + 1) / 2;
+            const int na = (1 << (lk - h)) + 1, nb = 1 << h;
+            e = plan_malloc(p, &p->d_rtwf, sizeof(double2) * (size_t)(na + nb + 4), zl::kInjectAlloc);
+            if (e != hipSuccess) return cleanup(fail(ZLFFT_ERR_ALLOC, "rfft twiddle hipMalloc failed"));
+            p->rtw_h = h;
+            p->rtw_b = na;
+            p->rtw_s = na + nb + 1;  // the W^{N/32} entry; W^{N/64} two further on
+            gen_macro_twiddles<<<(na + 255) / 256, 256, 0, p->stream>>>(p->d_rtwf, (int)order, h, na);
+            gen_macro_twiddles<<<(nb + 255) / 256, 256, 0, p->stream>>>(p->d_rtwf + na, (int)order, 0, nb);
+            gen_macro_twiddles<<<1, 256, 0, p->stream>>>(p->d_rtwf + na + nb, (int)order, (int)order - 5, 2);
+            gen_macro_twiddles<<<1, 256, 0, p->stream>>>(p->d_rtwf + na + nb + 2, (int)order, (int)order - 6, 2);
+            e = hipGetLastError();
+            if (e != hipSuccess) return cleanup(hip_fail(e, "rfft twiddle generation"));
+        }
     }
     e = hipStreamSynchronize(p->stream);
     if (e != hipSuccess) return cleanup(hip_fail(e, "plan synchronize"));
@@ -997,6 +1022,7 @@ int zlfft_plan_destroy(zlfft_plan p) {
     if (p->d_sink) (void)hipFree(p->d_sink);
     if (p->d_twm) (void)hipFree(p->d_twm);
     if (p->d_rtw) (void)hipFree(p->d_rtw);
+    if (p->d_rtwf) (void)hipFree(p->d_rtwf);
     for (auto& sl : p->slot) {
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.stream_out) (void)hipStreamSynchronize(sl.stream_out);

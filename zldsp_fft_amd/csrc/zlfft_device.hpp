@@ -1131,15 +1131,12 @@ fft_stream_kernel(KArgs a) {
 #define ZL_MACRO_NT 1
 #endif
 // RFFT twiddles of the fused real passes.  The first pass of a backward
-// transform loads them with the data (1; measured on 2^22-2^23-point batches,
+// transform takes them with the data (measured on 2^22-2^23-point batches,
 // orders 16-22: 1-6% faster than after the mirror exchange).  The last pass of
-// a forward transform loads them right after its sub-FFT, ahead of the
-// exchange's barriers: 0.3-6% faster than after the exchange, while loading
+// a forward transform takes them right after its sub-FFT, ahead of the
+// exchange's barriers: 0.3-6% faster than after the exchange, while taking
 // them with the data or before the sub-FFT was 4-9% slower (registers held
 // across the sub-FFT cost occupancy).
-#ifndef ZL_MACRO_RTW_EARLY_C2R
-#define ZL_MACRO_RTW_EARLY_C2R 1
-#endif
 
 // Arguments of one HBM pass of a large transform.
 struct MArgs {
@@ -1163,6 +1160,13 @@ struct MArgs {
     // signal[b hop + r] (in0 = signal), times window[r] (null: none)
     long long hop;
     const void* window;
+    // FP32 real kinds: RFFT twiddles from FP64 factors instead of the table,
+    // double2 W_{2N}^e = (cos, -sin): A[e >> rtwh] at rtwf, B[e & (2^rtwh - 1)]
+    // at rtwf + rtwb, and the steps W^{N/16}, W^{N/32} at rtwf + rtws
+    const void* rtwf;
+    int rtwh;
+    int rtwb;
+    int rtws;
 };
 
 // One pass of a multi-pass Stockham DIT transform of N = 2^logn points:
@@ -1294,16 +1298,43 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
     // twiddle rtw[k - 1] (32-bit indices: N < 2^31)
     const CX* rtw = static_cast<const CX*>(a.rtw);
     const int ni = (int)n, ii = (int)i, ci = (int)cols;
-    constexpr bool rtw_early = PAIR && FIRST && ZL_MACRO_RTW_EARLY_C2R;
+    constexpr bool rtw_early = PAIR && FIRST;
     CX rw[rtw_early ? R / 2 : 1];
-    auto rtw_of = [&](int j) -> CX {
-        const int k = ii + (t + TT * j) * ci;
-        return rtw[k > 0 ? k - 1 : 0];
-    };
-    if constexpr (rtw_early) {
+    // the twiddles of this lane's pairs, w[j] for k_j = k_0 + j N/R.  FP32:
+    // W^{k_0} from the FP64 factors, then one FP64 complex product by W^{N/R}
+    // per j, rounded to float as (0.5 cos, 0.5 sin + 0.5): the table's value
+    // but for ties closer than the recurrence's error (R/2 products, about
+    // 1e-15).  No lane reads the N/2-entry table, which for a single 2^22-point
+    // transform is half the size of its input.  FP64: the table.
+    auto twiddles = [&](CX (&w)[R / 2]) {
+        if constexpr (sizeof(CX) == 8) {
+            const double2* fa = static_cast<const double2*>(a.rtwf);
+            const int k0 = ii + t * ci;
+            const double2 wa = fa[k0 >> a.rtwh], wb = fa[a.rtwb + (k0 & ((1 << a.rtwh) - 1))];
+            const double2 st = fa[a.rtws + (R == 16 ? 0 : 2)];  // W^{N/(2R)}: R = 16 or 32
+            double c = wa.x * wb.x - wa.y * wb.y;
+            double sn = wa.x * wb.y + wa.y * wb.x;  // -sin
 #pragma unroll
-        for (int j = 0; j < R / 2; ++j) rw[j] = rtw_of(j);
-    }
+            for (int j = 0; j < R / 2; ++j) {
+                w[j] = T::mk((float)(0.5 * c), (float)(0.5 * -sn + 0.5));
+                const double c2 = c * st.x - sn * st.y;
+                sn = c * st.y + sn * st.x;
+                c = c2;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < R / 2; ++j) {
+                const int k = ii + (t + TT * j) * ci;
+                w[j] = rtw[k > 0 ? k - 1 : 0];
+            }
+        }
+    };
+    // bin N/2: W^{N/2} of the real size 2N is (0, -1), so (wc, ws) = (0, 1)
+    auto tw_half = [&]() -> CX {
+        if constexpr (sizeof(CX) == 8) return T::mk(S(0), S(1));
+        else return rtw[ni / 2 - 1];
+    };
+    if constexpr (rtw_early) twiddles(rw);
     if constexpr (FIRST && RK == C2R) {
         // RFFT pre-twiddle (reference src/common/zldsp_fft_common_execute.hpp:398-468):
         // Z[0] = ((X0 + XN) / 2, (X0 - XN) / 2) from the real parts of bins 0
@@ -1328,7 +1359,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
             } else {
                 CX* pm = mlds + GL::pad(ii == 0 ? L - q : L - 1 - q);
                 const CX x2 = *pm;
-                const CX w = rtw_early ? rw[rtw_early ? j : 0] : rtw_of(j);
+                const CX w = rw[rtw_early ? j : 0];
                 const S si = x1.y + x2.y, dr = x1.x - x2.x;
                 v[j] = T::mk(T::fma(-dr, w.y, T::fma(-si, w.x, x1.x)), T::fma(dr, w.x, T::fma(-si, w.y, x1.y)));
                 *pm = T::mk(T::fma(dr, w.y, T::fma(si, w.x, x2.x)), T::fma(dr, w.x, T::fma(-si, w.y, x2.y)));
@@ -1338,7 +1369,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         // mirror: the pair formula's second value, as in the reference
         CX zh = v[R / 2];
         if (ii == 0 && t == 0) {
-            const CX w = rtw[ni / 2 - 1];
+            const CX w = tw_half();
             const S si = zh.y + zh.y, dr = zh.x - zh.x;
             zh = T::mk(T::fma(dr, w.y, T::fma(si, w.x, zh.x)), T::fma(dr, w.x, T::fma(-si, w.y, zh.y)));
         }
@@ -1404,8 +1435,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         // (rows q < L/2, k < N/2): Z[k] is its own, Z[N - k] comes from the
         // mirror lane's rows q >= L/2, the only rows that go through LDS.
         CX w2[R / 2];  // twiddles of the pairs, in flight during the exchange
-#pragma unroll
-        for (int j = 0; j < R / 2; ++j) w2[j] = rtw_of(j);
+        twiddles(w2);
         __syncthreads();  // the sub-FFT's LDS reads done
 #pragma unroll
         for (int j = R / 2; j < R; ++j) slds[GL::pad(t + TT * j)] = v[j];
@@ -1446,7 +1476,7 @@ __global__ void __launch_bounds__(C * GL::T) fft_macro_pass(MArgs a) {
         // its value is the pair formula's second bin, as in the reference
         if (ii == 0 && t == 0) {
             const CX z = v[R / 2];
-            const CX w = rtw[ni / 2 - 1];
+            const CX w = tw_half();
             const S si = z.y + z.y, dr = z.x - z.x;
             put(ni / 2, T::fma(dr, w.y, T::fma(-si, w.x, z.x)), T::fma(-dr, w.x, T::fma(-si, w.y, z.y)), false);
         }
