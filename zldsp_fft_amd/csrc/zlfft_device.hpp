@@ -1162,7 +1162,8 @@ struct MArgs {
     const void* window;
     // FP32 real kinds: RFFT twiddles from FP64 factors instead of the table,
     // double2 W_{2N}^e = (cos, -sin): A[e >> rtwh] at rtwf, B[e & (2^rtwh - 1)]
-    // at rtwf + rtwb, and the steps W^{N/16}, W^{N/32} at rtwf + rtws
+    // at rtwf + rtwb, and the steps W^{N/16} at rtwf + rtws, W^{N/32} at
+    // rtwf + rtws + 2
     const void* rtwf;
     int rtwh;
     int rtwb;
